@@ -1,0 +1,589 @@
+// Host side of the MI355X curl-SVD + bi-anisotropic material library: plans,
+// launch sequences, CG (fncinvB) and Lanczos, behind the C-ABI of
+// include/fame_abi.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fame_abi.h"
+#include "fame_kernels.hpp"
+
+using namespace fame;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return set_err(FAME_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+#define CHECK_LAUNCH()                                                                         \
+  do {                                                                                         \
+    hipError_t e_ = hipGetLastError();                                                         \
+    if (e_ != hipSuccess) return set_err(FAME_ERR_HIP, std::string("launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define RET_IF(x)            \
+  do {                       \
+    int rc_ = (x);           \
+    if (rc_ != FAME_OK) return rc_; \
+  } while (0)
+
+constexpr int kBlk = 256;
+
+// Factor n into radices (4 first, then 2, 3, 5, 7, 11, 13).
+bool factor_radices(int n, std::vector<int>& r) {
+  r.clear();
+  int m = n;
+  while (m % 4 == 0) { r.push_back(4); m /= 4; }
+  while (m % 2 == 0) { r.push_back(2); m /= 2; }
+  for (int p : {3, 5, 7, 11, 13})
+    while (m % p == 0) { r.push_back(p); m /= p; }
+  if (m != 1) return false;
+  if ((int)r.size() > kMaxStages) return false;
+  if (r.empty()) r.push_back(1);
+  return true;
+}
+
+}  // namespace
+
+struct fame_plan {
+  int n1 = 0, n2 = 0, n3 = 0, nb = 0;
+  long long N = 0;
+  hipStream_t stream = nullptr;      // stream in use
+  hipStream_t own_stream = nullptr;  // stream created by the plan
+  Geo geo{};
+  double delta[3] = {0, 0, 0};
+  std::vector<double> theta;  // [nb][3]
+  int n_mat = 0;
+  // device tables
+  double2* d_tw = nullptr;    // all twiddle tables
+  int* d_perm = nullptr;      // all digit-reversal tables
+  double2* d_lam = nullptr;   // [nb][n1+n2+n3]
+  double2* d_tws = nullptr;   // [nb][n1+n2+n3]  Bloch twist factors
+  double2* d_omega = nullptr; // [nb][3]
+  unsigned char* d_mat_id = nullptr;
+  double2* d_mat = nullptr;   // [n_mat][27]
+  // scratch
+  double2* d_w1 = nullptr;    // [nb][6N]
+  double2* d_w2 = nullptr;    // [nb][6N]
+  double2* d_part = nullptr;  // reduction partials
+  double2* d_red = nullptr;   // [nb] reduced values
+  int nblk_red = 0;
+  // CG scratch
+  double2* d_r = nullptr;
+  double2* d_p = nullptr;
+  double2* d_q = nullptr;
+  CgState* d_cg = nullptr;
+  int tab_len = 0;            // n1+n2+n3
+  // per-stage timing of fncB (HIP events between the five kernels), off by default
+  bool prof = false;
+  hipEvent_t ev[6] = {};
+  double stage_ms[5] = {0, 0, 0, 0, 0};
+  long long prof_calls = 0;
+};
+
+namespace {
+
+int build_axis(int n, std::vector<double2>& tw, std::vector<int>& perm, AxisPlan& ap) {
+  std::vector<int> r;
+  if (!factor_radices(n, r))
+    return set_err(FAME_ERR_UNSUPPORTED, "grid length " + std::to_string(n) + " has a prime factor > 13");
+  ap.n = n;
+  ap.nstage = (n == 1) ? 0 : (int)r.size();
+  int L = n;
+  for (int s = 0; s < ap.nstage; ++s) {
+    ap.radix[s] = r[s];
+    ap.blen[s] = L;
+    L /= r[s];
+  }
+  ap.tw_off = (int)tw.size();
+  ap.perm_off = (int)perm.size();
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int t = 0; t < n; ++t) {
+    const long double a = two_pi * (long double)t / (long double)n;
+    tw.push_back(make_double2((double)cosl(a), (double)(-sinl(a))));
+  }
+  // frequency k sits at position sum_s q_s m_s (see fame_fft.hpp)
+  for (int k = 0; k < n; ++k) {
+    int kk = k, pos = 0, Ls = n;
+    for (int s = 0; s < ap.nstage; ++s) {
+      const int rs = ap.radix[s];
+      const int q = kk % rs;
+      kk /= rs;
+      const int m = Ls / rs;
+      pos += q * m;
+      Ls = m;
+    }
+    perm.push_back(pos);
+  }
+  return FAME_OK;
+}
+
+// lambda_d(j) = (exp(i(2 theta(n,j+1) + theta_d/n)) - 1)/delta_d (lambda_QuasiPeriodic.m:3),
+// S_d(j) = exp(i j theta_d/n) (S_sDFT.m:2), omega_d = exp(i theta_d/n).
+void fill_k_tables(const fame_plan* P, std::vector<double2>& lam, std::vector<double2>& tws,
+                   std::vector<double2>& om) {
+  const int dims[3] = {P->n1, P->n2, P->n3};
+  lam.assign((size_t)P->nb * P->tab_len, make_double2(0, 0));
+  tws.assign((size_t)P->nb * P->tab_len, make_double2(0, 0));
+  om.assign((size_t)P->nb * 3, make_double2(0, 0));
+  for (int b = 0; b < P->nb; ++b) {
+    int off = 0;
+    for (int d = 0; d < 3; ++d) {
+      const int m = dims[d];
+      const double th = P->theta[3 * b + d];
+      for (int j = 0; j < m; ++j) {
+        const double thm = (double)j * M_PI / (double)m;  // DistOp.theta(m, j+1)
+        const double arg = 2.0 * thm + th / (double)m;
+        lam[(size_t)b * P->tab_len + off + j] =
+            make_double2((std::cos(arg) - 1.0) / P->delta[d], std::sin(arg) / P->delta[d]);
+        const double as = (double)j * th / (double)m;
+        tws[(size_t)b * P->tab_len + off + j] = make_double2(std::cos(as), std::sin(as));
+      }
+      om[3 * b + d] = make_double2(std::cos(th / m), std::sin(th / m));
+      off += m;
+    }
+  }
+}
+
+int upload_k_tables(fame_plan* P) {
+  std::vector<double2> lam, tws, om;
+  fill_k_tables(P, lam, tws, om);
+  HIP_TRY(hipMemcpyAsync(P->d_lam, lam.data(), lam.size() * sizeof(double2), hipMemcpyHostToDevice, P->stream));
+  HIP_TRY(hipMemcpyAsync(P->d_tws, tws.data(), tws.size() * sizeof(double2), hipMemcpyHostToDevice, P->stream));
+  HIP_TRY(hipMemcpyAsync(P->d_omega, om.data(), om.size() * sizeof(double2), hipMemcpyHostToDevice, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  return FAME_OK;
+}
+
+size_t zpass_lds(const fame_plan* P) { return (size_t)3 * kZTile * P->n3 * sizeof(double2); }
+size_t xypass_lds(const fame_plan* P) { return (size_t)P->n1 * P->n2 * sizeof(double2); }
+
+unsigned grid1d(long long len) {
+  long long g = (len + kBlk - 1) / kBlk;
+  if (g > 2048) g = 2048;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// ---- launch helpers -------------------------------------------------------
+int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs, int nslot,
+          int grp_base) {
+  const dim3 grid((P->geo.n12 + kZTile - 1) / kZTile, nslot, P->nb);
+  const size_t lds = zpass_lds(P);
+  const double sc = 1.0 / std::sqrt((double)P->n3);
+  if (dir > 0) {
+    if (mode == ZM_MIX_IN)
+      hipLaunchKernelGGL((k_zpass<1, ZM_MIX_IN>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+    else
+      hipLaunchKernelGGL((k_zpass<1, ZM_PLAIN>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+  } else {
+    if (mode == ZM_MIX_OUT)
+      hipLaunchKernelGGL((k_zpass<-1, ZM_MIX_OUT>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+    else
+      hipLaunchKernelGGL((k_zpass<-1, ZM_PLAIN>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+  }
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs, int nf) {
+  const dim3 grid(P->n3, nf, P->nb);
+  const size_t lds = xypass_lds(P);
+  const double sc = 1.0 / std::sqrt((double)P->n1 * (double)P->n2);
+#define XY_LAUNCH(D, T)                                                                                   \
+  hipLaunchKernelGGL((k_xypass<D, T>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs, P->d_tws, \
+                     P->tab_len, P->d_tw, P->d_perm, sc)
+  if (dir > 0) {
+    if (twist == 1) XY_LAUNCH(1, 1); else XY_LAUNCH(1, 0);
+  } else {
+    if (twist == 2) XY_LAUNCH(-1, 2); else XY_LAUNCH(-1, 0);
+  }
+#undef XY_LAUNCH
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs) {
+  const int tx = (P->n1 + kMatTX - 1) / kMatTX, ty = (P->n2 + kMatTY - 1) / kMatTY;
+  const int tz = (P->n3 + kMatTZ - 1) / kMatTZ;
+  hipLaunchKernelGGL(k_material, dim3(tx * ty, tz, P->nb), dim3(kBlk), 0, P->stream, P->geo, in, ibs, out, obs,
+                     P->d_mat_id, P->d_mat, P->n_mat, P->d_omega);
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+int twist(fame_plan* P, const double2* in, double2* out, int ncomp, int conj_twist) {
+  hipLaunchKernelGGL(k_twist, dim3(grid1d(P->N), P->nb), dim3(kBlk), 0, P->stream, P->geo, in, out, ncomp,
+                     P->d_tws, P->tab_len, conj_twist);
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+int do_fncB(fame_plan* P, const double2* x, double2* y) {
+  const long long N = P->N;
+  const bool pr = P->prof;
+  if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
+  RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0));
+  if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
+  RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
+  if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
+  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N));
+  if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
+  RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
+  if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
+  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0));
+  if (pr) {
+    HIP_TRY(hipEventRecord(P->ev[5], P->stream));
+    HIP_TRY(hipEventSynchronize(P->ev[5]));
+    for (int s = 0; s < 5; ++s) {
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, P->ev[s], P->ev[s + 1]));
+      P->stage_ms[s] += ms;
+    }
+    P->prof_calls += 1;
+  }
+  return FAME_OK;
+}
+
+int do_fncA(fame_plan* P, const double2* x, double2* y) {
+  hipLaunchKernelGGL(k_fncA, dim3(grid1d(P->N), P->nb), dim3(kBlk), 0, P->stream, P->geo, x, y, P->d_lam,
+                     P->tab_len);
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+// out[b] = x[b]^H y[b] (device, P->d_red or given)
+int do_dot(fame_plan* P, const double2* x, long long xbs, const double2* y, long long ybs, long long len, int nb,
+           double2* out) {
+  const int nblk = std::min<int>(P->nblk_red, (int)grid1d(len));
+  hipLaunchKernelGGL(k_dot_part, dim3(nblk, nb), dim3(kBlk), 0, P->stream, x, xbs, y, ybs, len, P->d_part);
+  CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_reduce_part, dim3(nb), dim3(kBlk), 0, P->stream, P->d_part, nblk, out);
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+int fill_zero(fame_plan* P, double2* y, long long len) {
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid1d(len)), dim3(kBlk), 0, P->stream, y, len);
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+int copy(fame_plan* P, const double2* x, double2* y, long long len) {
+  hipLaunchKernelGGL(k_copy, dim3(grid1d(len)), dim3(kBlk), 0, P->stream, x, y, len);
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+int do_cg(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int* iters, double* relres) {
+  const long long L = 4 * P->N;
+  const long long tot = L * P->nb;
+  const int nb = P->nb;
+  RET_IF(fill_zero(P, x, tot));
+  RET_IF(copy(P, b, P->d_r, tot));
+  RET_IF(copy(P, b, P->d_p, tot));
+  RET_IF(do_dot(P, b, L, b, L, L, nb, P->d_red));
+  hipLaunchKernelGGL(k_cg_init, dim3((nb + 63) / 64), dim3(64), 0, P->stream, P->d_red, P->d_cg, nb, tol);
+  CHECK_LAUNCH();
+  const int nblk = std::min<int>(P->nblk_red, (int)grid1d(L));
+  std::vector<CgState> hs(nb);
+  int it = 0;
+  bool done = false;
+  const int check_every = 4;
+  while (it < maxit && !done) {
+    RET_IF(do_fncB(P, P->d_p, P->d_q));
+    RET_IF(do_dot(P, P->d_p, L, P->d_q, L, L, nb, P->d_red));
+    hipLaunchKernelGGL(k_cg_alpha, dim3((nb + 63) / 64), dim3(64), 0, P->stream, P->d_red, P->d_cg, nb);
+    hipLaunchKernelGGL(k_cg_update, dim3(nblk, nb), dim3(kBlk), 0, P->stream, x, P->d_r, P->d_p, P->d_q, L, P->d_cg,
+                       P->d_part);
+    hipLaunchKernelGGL(k_cg_beta, dim3(nb), dim3(kBlk), 0, P->stream, P->d_part, nblk, P->d_cg);
+    hipLaunchKernelGGL(k_cg_p, dim3(grid1d(L), nb), dim3(kBlk), 0, P->stream, P->d_p, P->d_r, L, P->d_cg);
+    CHECK_LAUNCH();
+    ++it;
+    if (it % check_every == 0 || it == maxit) {
+      HIP_TRY(hipMemcpyAsync(hs.data(), P->d_cg, nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
+      HIP_TRY(hipStreamSynchronize(P->stream));
+      done = true;
+      for (int i = 0; i < nb; ++i)
+        if (hs[i].active != 0.0) done = false;
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(hs.data(), P->d_cg, nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  bool all = true;
+  for (int i = 0; i < nb; ++i) {
+    if (iters) iters[i] = (int)hs[i].iters;
+    if (relres) relres[i] = hs[i].bnorm2 > 0 ? std::sqrt(hs[i].rho / hs[i].bnorm2) : 0.0;
+    if (hs[i].active != 0.0) all = false;
+  }
+  if (!all) return set_err(FAME_ERR_NOT_CONVERGED, "fncinvB: CG reached maxit before tol");
+  return FAME_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* fame_version(void) { return "fame-mi355x 0.1 (gfx950, FP64)"; }
+
+const char* fame_last_error(void) { return g_err.c_str(); }
+
+int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const double* theta, const double* delta,
+                     int n_mat, const double* mat_table, const unsigned char* mat_id) {
+  if (!plan) return set_err(FAME_ERR_ARG, "plan pointer is NULL");
+  *plan = nullptr;
+  if (n1 < 1 || n2 < 1 || n3 < 1 || nb < 1) return set_err(FAME_ERR_ARG, "grid dims and nb must be >= 1");
+  if (!theta || !delta) return set_err(FAME_ERR_ARG, "theta and delta are required");
+  if (n_mat < 1 || n_mat > kMaxMat || !mat_table)
+    return set_err(FAME_ERR_ARG, "need 1 <= n_mat <= " + std::to_string(kMaxMat) + " and a material table");
+  if ((long long)n1 * n2 * (long long)sizeof(double2) > 160 * 1024)
+    return set_err(FAME_ERR_UNSUPPORTED, "n1*n2 too large for one xy-plane in LDS");
+  if ((size_t)3 * kZTile * n3 * sizeof(double2) > 160 * 1024)
+    return set_err(FAME_ERR_UNSUPPORTED, "n3 too large for the z-pencil tile in LDS");
+  const long long N = (long long)n1 * n2 * n3;
+  if (mat_id) {
+    for (long long p = 0; p < N; ++p)
+      if (mat_id[p] >= n_mat) return set_err(FAME_ERR_ARG, "mat_id entry >= n_mat");
+  }
+  fame_plan* P = new fame_plan();
+  P->n1 = n1; P->n2 = n2; P->n3 = n3; P->nb = nb; P->N = N; P->n_mat = n_mat;
+  P->tab_len = n1 + n2 + n3;
+  P->theta.assign(theta, theta + 3 * nb);
+  for (int d = 0; d < 3; ++d) P->delta[d] = delta[d];
+  P->geo.n1 = n1; P->geo.n2 = n2; P->geo.n3 = n3; P->geo.n12 = n1 * n2; P->geo.N = N;
+  std::vector<double2> tw;
+  std::vector<int> perm;
+  std::map<int, int> seen;
+  const int dims[3] = {n1, n2, n3};
+  for (int d = 0; d < 3; ++d) {
+    if (seen.count(dims[d])) {
+      P->geo.ax[d] = P->geo.ax[seen[dims[d]]];
+      continue;
+    }
+    int rc = build_axis(dims[d], tw, perm, P->geo.ax[d]);
+    if (rc != FAME_OK) { delete P; return rc; }
+    seen[dims[d]] = d;
+  }
+  auto fail = [&](int rc) { fame_plan_destroy(P); return rc; };
+#define PLAN_HIP(call)                                                                              \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess) return fail(set_err(FAME_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+  PLAN_HIP(hipStreamCreateWithFlags(&P->own_stream, hipStreamNonBlocking));
+  P->stream = P->own_stream;
+  PLAN_HIP(hipMalloc(&P->d_tw, tw.size() * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_perm, perm.size() * sizeof(int)));
+  PLAN_HIP(hipMemcpy(P->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+  PLAN_HIP(hipMemcpy(P->d_perm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
+  PLAN_HIP(hipMalloc(&P->d_lam, (size_t)nb * P->tab_len * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_tws, (size_t)nb * P->tab_len * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_omega, (size_t)nb * 3 * sizeof(double2)));
+  // material: symmetrise eps and mu as 1/2 (X + X^H) (the symmetrisation of N_int, M_int at
+  // Interpolation_Material_Matrix_Mult_Point_vertex.m:22,24 acts on the vertex tensors like this)
+  std::vector<double2> mt((size_t)n_mat * kMatWords);
+  for (int m = 0; m < n_mat; ++m) {
+    const double* src = mat_table + (size_t)m * kMatWords * 2;
+    auto at = [&](int blk, int i, int j) { return make_double2(src[2 * (9 * blk + 3 * i + j)], src[2 * (9 * blk + 3 * i + j) + 1]); };
+    for (int blk = 0; blk < 3; ++blk)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double2 v = at(blk, i, j);
+          if (blk < 2) {
+            const double2 t = at(blk, j, i);
+            v = make_double2(0.5 * (v.x + t.x), 0.5 * (v.y - t.y));
+          }
+          mt[(size_t)m * kMatWords + 9 * blk + 3 * i + j] = v;
+        }
+  }
+  PLAN_HIP(hipMalloc(&P->d_mat, mt.size() * sizeof(double2)));
+  PLAN_HIP(hipMemcpy(P->d_mat, mt.data(), mt.size() * sizeof(double2), hipMemcpyHostToDevice));
+  PLAN_HIP(hipMalloc(&P->d_mat_id, (size_t)N));
+  if (mat_id)
+    PLAN_HIP(hipMemcpy(P->d_mat_id, mat_id, (size_t)N, hipMemcpyHostToDevice));
+  else
+    PLAN_HIP(hipMemset(P->d_mat_id, 0, (size_t)N));
+  PLAN_HIP(hipMalloc(&P->d_w1, (size_t)nb * 6 * N * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_w2, (size_t)nb * 6 * N * sizeof(double2)));
+  P->nblk_red = 1024;
+  PLAN_HIP(hipMalloc(&P->d_part, (size_t)nb * P->nblk_red * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_red, (size_t)nb * 2 * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_cg, (size_t)nb * sizeof(CgState)));
+  // LDS above 64 KB needs an explicit opt-in per kernel
+  const int lds_max = (int)std::max(xypass_lds(P), zpass_lds(P));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_xypass<1, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_xypass<1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_xypass<-1, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_xypass<-1, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<1, ZM_MIX_IN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<1, ZM_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_MIX_OUT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+#undef PLAN_HIP
+  int rc = upload_k_tables(P);
+  if (rc != FAME_OK) return fail(rc);
+  *plan = P;
+  return FAME_OK;
+}
+
+int fame_plan_set_theta(fame_plan* P, const double* theta) {
+  if (!P || !theta) return set_err(FAME_ERR_ARG, "NULL argument");
+  P->theta.assign(theta, theta + 3 * P->nb);
+  return upload_k_tables(P);
+}
+
+int fame_plan_destroy(fame_plan* P) {
+  if (!P) return FAME_OK;
+  if (P->stream) (void)hipStreamSynchronize(P->stream);
+  void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
+                  P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (hipEvent_t e : P->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (P->own_stream) {
+    (void)hipStreamSynchronize(P->own_stream);
+    (void)hipStreamDestroy(P->own_stream);
+  }
+  delete P;
+  return FAME_OK;
+}
+
+int fame_sync(fame_plan* P) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  return FAME_OK;
+}
+
+void* fame_plan_stream(fame_plan* P) { return P ? (void*)P->stream : nullptr; }
+
+int fame_plan_set_stream(fame_plan* P, void* stream) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  P->stream = stream ? (hipStream_t)stream : P->own_stream;
+  return FAME_OK;
+}
+
+int fame_plan_profile(fame_plan* P, int enable) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  if (enable && !P->ev[0])
+    for (auto& e : P->ev) HIP_TRY(hipEventCreate(&e));
+  P->prof = enable != 0;
+  for (double& v : P->stage_ms) v = 0;
+  P->prof_calls = 0;
+  return FAME_OK;
+}
+
+int fame_plan_stage_times(fame_plan* P, double* ms, long long* calls) {
+  if (!P || !ms) return set_err(FAME_ERR_ARG, "NULL argument");
+  for (int s = 0; s < 5; ++s) ms[s] = P->stage_ms[s];
+  if (calls) *calls = P->prof_calls;
+  return FAME_OK;
+}
+
+int fame_MtxVecMul_FFT(fame_plan* P, int istrans, const void* in, void* out) {
+  if (!P || !in || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  const long long N = P->N;
+  const double2* x = (const double2*)in;
+  double2* y = (double2*)out;
+  if (istrans == 0) {
+    RET_IF(zpass(P, +1, ZM_PLAIN, x, 3 * N, P->d_w1, 3 * N, 1, 0));
+    RET_IF(xypass(P, +1, 1, P->d_w1, 3 * N, y, 3 * N, 3));
+  } else if (istrans == 1) {
+    RET_IF(xypass(P, -1, 2, x, 3 * N, P->d_w1, 3 * N, 3));
+    RET_IF(zpass(P, -1, ZM_PLAIN, P->d_w1, 3 * N, y, 3 * N, 1, 0));
+  } else {
+    return set_err(FAME_ERR_ARG, "istrans must be 0 ('notranspose') or 1 ('transpose')");
+  }
+  return FAME_OK;
+}
+
+static int pr_qr(fame_plan* P, int grp, const void* in, void* out) {
+  if (!P || !in || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  const long long N = P->N;
+  RET_IF(zpass(P, +1, ZM_MIX_IN, (const double2*)in, 2 * N, P->d_w1, 3 * N, 1, grp));
+  RET_IF(xypass(P, +1, 1, P->d_w1, 3 * N, (double2*)out, 3 * N, 3));
+  return FAME_OK;
+}
+
+static int prs_qrs(fame_plan* P, int grp, const void* in, void* out) {
+  if (!P || !in || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  const long long N = P->N;
+  RET_IF(xypass(P, -1, 2, (const double2*)in, 3 * N, P->d_w1, 3 * N, 3));
+  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w1, 3 * N, (double2*)out, 2 * N, 1, grp));
+  return FAME_OK;
+}
+
+int fame_Pr(fame_plan* P, const void* in, void* out) { return pr_qr(P, 0, in, out); }
+int fame_Qr(fame_plan* P, const void* in, void* out) { return pr_qr(P, 1, in, out); }
+int fame_Prs(fame_plan* P, const void* in, void* out) { return prs_qrs(P, 0, in, out); }
+int fame_Qrs(fame_plan* P, const void* in, void* out) { return prs_qrs(P, 1, in, out); }
+
+int fame_fncA(fame_plan* P, const void* in, void* out) {
+  if (!P || !in || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  return do_fncA(P, (const double2*)in, (double2*)out);
+}
+
+int fame_fncB(fame_plan* P, const void* in, void* out) {
+  if (!P || !in || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  if (in == out) return set_err(FAME_ERR_ARG, "fncB: in and out must not alias");
+  return do_fncB(P, (const double2*)in, (double2*)out);
+}
+
+int fame_material_apply(fame_plan* P, const void* in, void* out) {
+  if (!P || !in || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  const long long N = P->N;
+  RET_IF(twist(P, (const double2*)in, P->d_w1, 6, 1));
+  RET_IF(material(P, P->d_w1, 6 * N, P->d_w2, 6 * N));
+  RET_IF(twist(P, P->d_w2, (double2*)out, 6, 0));
+  return FAME_OK;
+}
+
+int fame_dot(fame_plan* P, const void* x, const void* y, long long len, void* out) {
+  if (!P || !x || !y || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  return do_dot(P, (const double2*)x, len, (const double2*)y, len, len, P->nb, (double2*)out);
+}
+
+static int ensure_cg(fame_plan* P) {
+  const size_t bytes = (size_t)P->nb * 4 * P->N * sizeof(double2);
+  if (!P->d_r) HIP_TRY(hipMalloc(&P->d_r, bytes));
+  if (!P->d_p) HIP_TRY(hipMalloc(&P->d_p, bytes));
+  if (!P->d_q) HIP_TRY(hipMalloc(&P->d_q, bytes));
+  return FAME_OK;
+}
+
+int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit, int* iters, double* relres) {
+  if (!P || !in || !out) return set_err(FAME_ERR_ARG, "NULL argument");
+  if (in == out) return set_err(FAME_ERR_ARG, "fncinvB: in and out must not alias");
+  if (!(tol > 0) || maxit < 1) return set_err(FAME_ERR_ARG, "need tol > 0 and maxit >= 1");
+  RET_IF(ensure_cg(P));
+  return do_cg(P, (const double2*)in, (double2*)out, tol, maxit, iters, relres);
+}
+
+}  // extern "C"
+
+#include "fame_lanczos.inc"

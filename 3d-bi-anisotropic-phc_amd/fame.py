@@ -1,0 +1,298 @@
+"""Python mirror of the reference's operator interface over the C-ABI (``libfame.so``).
+
+The reference is MATLAB; its eigensolvers see the hot path through function
+handles with these signatures (names kept):
+
+* ``MtxVecMul_FFT(vec_in, plan, istrans)``  subroutines/gMtxVecMul_FFT.m:1
+* ``fnc.Pr / Qr / Prs / Qrs``                fun_Lanczos/Funhand_Generate_Lanzcos.m:79-82
+* ``fncA(x, plan)``                          fun_Lanczos/fncA.m:1
+* ``fncB(x, plan)``                          fun_Lanczos/fncB.m:1
+* ``fncinvB(x, plan, tol, maxit)``           fun_Lanczos/fncinvB.m:1
+* ``Lanczos(plan, k, tol, maxiter, p, v0)``  fun_Lanczos/Lanczos.m:1
+* ``Eigen_Restoration_Chiral_no_inv``        fun_no_inv/Eigen_Restoration_Chiral_no_inv.m:1
+
+Every call runs the HIP kernels of ``libfame.so`` on the GPU.  There is no CPU
+fallback: if the library is missing or no GPU is present the calls raise.
+
+Vectors are complex128 torch tensors on a CUDA (HIP) device, shaped
+``(nb, len)`` (or ``(len,)`` when ``nb == 1``), in the reference's MATLAB
+column-major order.  The plan runs on torch's current stream, so results are
+ordered with surrounding torch work without extra synchronisation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfame.so")
+
+FAME_OK = 0
+FAME_ERR_ARG = -1
+FAME_ERR_HIP = -2
+FAME_ERR_UNSUPPORTED = -3
+FAME_ERR_NOT_CONVERGED = -4
+
+_lib = None
+
+
+class FameError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fame error {code}: {msg}")
+        self.code = code
+
+
+class NotConverged(FameError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load ``libfame.so`` and declare the C-ABI signatures (include/fame_abi.h)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FameError(FAME_ERR_HIP, f"{path} is not built; run build() first (no CPU fallback exists)")
+    lib = ctypes.CDLL(path)
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    pd, pi_ = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "fame_version": ([], ctypes.c_char_p),
+        "fame_last_error": ([], ctypes.c_char_p),
+        "fame_plan_create": ([ctypes.POINTER(vp), i, i, i, i, pd, pd, i, pd, ctypes.c_void_p], i),
+        "fame_plan_set_theta": ([vp, pd], i),
+        "fame_plan_destroy": ([vp], i),
+        "fame_sync": ([vp], i),
+        "fame_plan_stream": ([vp], vp),
+        "fame_plan_set_stream": ([vp, vp], i),
+        "fame_MtxVecMul_FFT": ([vp, i, vp, vp], i),
+        "fame_Pr": ([vp, vp, vp], i),
+        "fame_Qr": ([vp, vp, vp], i),
+        "fame_Prs": ([vp, vp, vp], i),
+        "fame_Qrs": ([vp, vp, vp], i),
+        "fame_fncA": ([vp, vp, vp], i),
+        "fame_fncB": ([vp, vp, vp], i),
+        "fame_material_apply": ([vp, vp, vp], i),
+        "fame_fncinvB": ([vp, vp, vp, d, i, pi_, pd], i),
+        "fame_lanczos": ([vp, i, d, i, i, vp, d, i, vp, pd, pi_], i),
+        "fame_eigen_restoration": ([vp, i, vp, vp], i),
+        "fame_dot": ([vp, vp, vp, ctypes.c_longlong, vp], i),
+        "fame_plan_profile": ([vp, i], i),
+        "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points this wrapper binds."""
+    return [
+        "fame_version", "fame_last_error", "fame_plan_create", "fame_plan_set_theta", "fame_plan_destroy",
+        "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
+        "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
+        "fame_eigen_restoration", "fame_dot", "fame_plan_profile", "fame_plan_stage_times",
+    ]
+
+
+def _check(rc):
+    if rc == FAME_OK:
+        return
+    msg = _lib.fame_last_error().decode()
+    if rc == FAME_ERR_NOT_CONVERGED:
+        raise NotConverged(rc, msg)
+    raise FameError(rc, msg)
+
+
+def _dptr(arr):
+    """Pointer to a float64 C-contiguous array; the returned ctypes object keeps ``arr`` alive,
+    so ``arr`` must already be float64 and contiguous (no temporary copy is made here)."""
+    if arr.dtype != np.float64 or not arr.flags.c_contiguous:
+        raise TypeError("_dptr needs a C-contiguous float64 array")
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Plan:
+    """One grid + material with ``nb`` Bloch vectors; owns the device tables.
+
+    Parameters mirror what the reference's per-k setup consumes:
+    ``theta`` (nb,3) = k . a_d, ``delta`` (3,) mesh lengths, ``mat_table``
+    (n_mat, 3, 3, 3) complex = per material the N-, M-, J-slot tensors, and
+    ``mat_id`` (N,) uint8 per-vertex material index (x fastest).
+    """
+
+    def __init__(self, dims, theta, mat_table, mat_id=None, delta=None, device=None):
+        import torch
+
+        self.lib = load_library()
+        if not torch.cuda.is_available():
+            raise FameError(FAME_ERR_HIP, "no GPU visible: the HIP path has no CPU fallback")
+        self.torch = torch
+        self.device = torch.device("cuda" if device is None else device)
+        self.dims = tuple(int(x) for x in dims)
+        self.N = int(np.prod(self.dims))
+        th = np.ascontiguousarray(np.atleast_2d(np.asarray(theta, dtype=np.float64)))
+        self.nb = th.shape[0]
+        self.theta = th.copy()
+        self.delta = np.ascontiguousarray(
+            np.array([1.0 / d for d in self.dims]) if delta is None else np.asarray(delta, np.float64))
+        tab = np.ascontiguousarray(np.asarray(mat_table, dtype=np.complex128).reshape(-1, 27))
+        self.n_mat = tab.shape[0]
+        self._tab = tab
+        ids = None
+        if mat_id is not None:
+            ids = np.ascontiguousarray(np.asarray(mat_id, dtype=np.uint8).reshape(-1))
+            if ids.size != self.N:
+                raise ValueError("mat_id must have N entries")
+        self._ids = ids
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            rc = self.lib.fame_plan_create(
+                ctypes.byref(h), self.dims[0], self.dims[1], self.dims[2], self.nb, _dptr(th), _dptr(self.delta),
+                self.n_mat, ctypes.cast(tab.ctypes.data, ctypes.POINTER(ctypes.c_double)),
+                None if ids is None else ids.ctypes.data)
+        _check(rc)
+        self.h = h
+        _check(self.lib.fame_plan_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)))
+
+    # -- lifetime ---------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fame_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_theta(self, theta):
+        th = np.ascontiguousarray(np.atleast_2d(np.asarray(theta, dtype=np.float64)))
+        if th.shape != (self.nb, 3):
+            raise ValueError("theta must be (nb, 3)")
+        self.theta = th.copy()
+        _check(self.lib.fame_plan_set_theta(self.h, _dptr(th)))
+
+    def sync(self):
+        _check(self.lib.fame_sync(self.h))
+
+    STAGES = ("zpass_mix_in", "xypass_inv", "material", "xypass_fwd", "zpass_mix_out")
+
+    def profile(self, enable=True):
+        """Per-kernel HIP-event timing of fncB (measurement aid)."""
+        _check(self.lib.fame_plan_profile(self.h, int(bool(enable))))
+
+    def stage_times(self):
+        ms = (ctypes.c_double * 5)()
+        calls = ctypes.c_longlong()
+        _check(self.lib.fame_plan_stage_times(self.h, ms, ctypes.byref(calls)))
+        return dict(zip(self.STAGES, ms[:])), calls.value
+
+    # -- buffers ------------------------------------------------------------------
+    def empty(self, ncomp):
+        return self.torch.empty((self.nb, ncomp * self.N), dtype=self.torch.complex128, device=self.device)
+
+    def _as_dev(self, x, ncomp):
+        t = self.torch
+        if not isinstance(x, t.Tensor):
+            x = t.as_tensor(np.asarray(x, dtype=np.complex128))
+        x = x.to(device=self.device, dtype=t.complex128).contiguous()
+        if x.numel() != self.nb * ncomp * self.N:
+            raise ValueError(f"expected {self.nb} x {ncomp}N complex values, got {x.numel()}")
+        return x.reshape(self.nb, ncomp * self.N)
+
+    def _shape_out(self, y, like):
+        return y.reshape(like.shape) if like.dim() == 1 or like.shape == y.shape else y
+
+    def _call(self, fn, x, nin, nout, *extra):
+        xin = self._as_dev(x, nin)
+        y = self.empty(nout)
+        _check(fn(self.h, *extra, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr())))
+        return y.reshape(-1) if self.nb == 1 else y
+
+    # -- operators (MATLAB names) ------------------------------------------------
+    def MtxVecMul_FFT(self, vec_in, istrans):
+        """Psi/Phi ('notranspose') or Psis/Phis ('transpose') on [nb][3N] (gMtxVecMul_FFT.m:1-61)."""
+        flag = {"notranspose": 0, "transpose": 1}[istrans]
+        return self._call(self.lib.fame_MtxVecMul_FFT, vec_in, 3, 3, flag)
+
+    def Pr(self, x):
+        return self._call(self.lib.fame_Pr, x, 2, 3)
+
+    def Qr(self, x):
+        return self._call(self.lib.fame_Qr, x, 2, 3)
+
+    def Prs(self, y):
+        return self._call(self.lib.fame_Prs, y, 3, 2)
+
+    def Qrs(self, y):
+        return self._call(self.lib.fame_Qrs, y, 3, 2)
+
+    def fncA(self, x):
+        return self._call(self.lib.fame_fncA, x, 4, 4)
+
+    def fncB(self, x):
+        return self._call(self.lib.fame_fncB, x, 4, 4)
+
+    def material_apply(self, he):
+        """[M h + J^H e; J h + N e] on twisted real-space [h; e] (fncB.m:4-5)."""
+        return self._call(self.lib.fame_material_apply, he, 6, 6)
+
+    def fncinvB(self, x, tol=1e-12, maxit=10000, raise_on_maxit=True):
+        """CG solve of B y = x (fncinvB.m:4); returns (y, iters, relres)."""
+        xin = self._as_dev(x, 4)
+        y = self.empty(4)
+        iters = (ctypes.c_int * self.nb)()
+        relres = (ctypes.c_double * self.nb)()
+        rc = self.lib.fame_fncinvB(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                   float(tol), int(maxit), iters, relres)
+        if rc != FAME_ERR_NOT_CONVERGED or raise_on_maxit:
+            _check(rc)
+        y = y.reshape(-1) if self.nb == 1 else y
+        return y, np.array(iters[:]), np.array(relres[:])
+
+    def Lanczos(self, k, tol=1e-10, maxiter=10000, p=None, v0=None, cg_tol=1e-12, cg_maxit=10000):
+        """Thick-restart Lanczos (Lanczos.m:1-156); returns (ev [nb,k,4N], ew [nb,k], iters [nb])."""
+        p = 2 * k if p is None else p
+        if v0 is None:
+            v0 = self.torch.ones((self.nb, 4 * self.N), dtype=self.torch.complex128, device=self.device)
+        v = self._as_dev(v0, 4)
+        ev = self.torch.empty((self.nb, k, 4 * self.N), dtype=self.torch.complex128, device=self.device)
+        ew = np.zeros((self.nb, k))
+        iters = (ctypes.c_int * self.nb)()
+        rc = self.lib.fame_lanczos(self.h, int(k), float(tol), int(maxiter), int(p), ctypes.c_void_p(v.data_ptr()),
+                                   float(cg_tol), int(cg_maxit), ctypes.c_void_p(ev.data_ptr()),
+                                   ew.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), iters)
+        _check(rc)
+        return ev, ew, np.array(iters[:])
+
+    def Eigen_Restoration_Chiral_no_inv(self, ev):
+        """(e, h) fields from NFSEP eigenvectors (Eigen_Restoration_Chiral_no_inv.m:3-25).
+
+        ``ev`` is (nb, nvec, 4N); returns (nb, nvec, 6N) = [e; h]."""
+        t = self.torch
+        ev = ev.to(device=self.device, dtype=t.complex128).contiguous()
+        nvec = ev.shape[-2] if ev.dim() >= 2 else 1
+        out = t.empty((self.nb, nvec, 6 * self.N), dtype=t.complex128, device=self.device)
+        _check(self.lib.fame_eigen_restoration(self.h, int(nvec), ctypes.c_void_p(ev.data_ptr()),
+                                               ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def Sigma_r(self):
+        """Sigma_r = [sqrt(Lambda_q); sqrt(Lambda_q)] per item (SVD_Matrix_curl.m:74), on the host."""
+        out = []
+        for b in range(self.nb):
+            lam = [(np.exp(1j * (2 * np.arange(n) * np.pi / n + self.theta[b, d] / n)) - 1) / self.delta[d]
+                   for d, n in enumerate(self.dims)]
+            n1, n2, n3 = self.dims
+            lq = (np.abs(np.kron(np.ones(n3 * n2), lam[0])) ** 2
+                  + np.abs(np.kron(np.ones(n3), np.kron(lam[1], np.ones(n1)))) ** 2
+                  + np.abs(np.kron(lam[2], np.ones(n2 * n1))) ** 2)
+            out.append(np.concatenate([np.sqrt(lq), np.sqrt(lq)]))
+        return np.stack(out)

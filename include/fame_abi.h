@@ -1,0 +1,143 @@
+/* C-ABI of the MI355X (gfx950) curl-SVD + bi-anisotropic material matvec library.
+ *
+ * Drop-in for the operator handles the reference's eigensolvers call through
+ * MATLAB function handles (FAME-GPU/3D-Bi-Anisotropic-PhC).  Each entry point
+ * names the reference function (file:line) it replaces.
+ *
+ * Conventions
+ *   - Vectors are complex128, interleaved (re, im), in the reference's MATLAB
+ *     column-major order: grid point p = i1 + n1*(i2 + n2*i3).  No permutation
+ *     at the boundary.
+ *   - All vector pointers are DEVICE pointers owned by the caller.  A plan holds
+ *     `nb` Bloch vectors (k-points) on one grid and one material; vectors are
+ *     batched as [nb][len] contiguous.  nb = 1 is the reference's plain path.
+ *     Lengths: 3-component field 3N, Pr/Qr coefficient vector 2N, NFSEP vector
+ *     4N = [x_a (2N); x_b (2N)], N = n1*n2*n3.
+ *   - Calls are asynchronous on the plan's HIP stream; fame_sync() waits.  Solver
+ *     calls (fame_fncinvB, fame_lanczos) return after their result is on the
+ *     device.  Distinct plans may be used from distinct threads.
+ *   - Return value: FAME_OK (0) or a negative error code; fame_last_error()
+ *     returns the message of the calling thread's last error.
+ *   - Grid lengths must factor into primes <= 13; every axis holds at most
+ *     10240 points per xy-plane (n1*n2 <= 10240) and n3 <= 208.
+ */
+#ifndef FAME_ABI_H
+#define FAME_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FAME_OK 0
+#define FAME_ERR_ARG (-1)
+#define FAME_ERR_HIP (-2)
+#define FAME_ERR_UNSUPPORTED (-3)
+#define FAME_ERR_NOT_CONVERGED (-4)
+
+typedef struct fame_plan fame_plan;
+
+/* Version string of the library build. */
+const char* fame_version(void);
+
+/* Message of the last error raised on the calling thread ("" if none). */
+const char* fame_last_error(void);
+
+/* Create a plan: grid, Bloch phases, mesh lengths and material.
+ *   theta      [nb][3]  theta_d = k . a_d
+ *                       (band_structure_woodpile_Lanczos_n60_gamma0p2.m:54-56)
+ *   delta      [3]      mesh lengths (Computational_settings_Lanczos.m:38)
+ *   mat_table  [n_mat][27] complex (54 doubles): per material the 3x3 N-slot
+ *              (inverse permittivity, covariant), M-slot (inverse permeability)
+ *              and J-slot (inverse Xi) tensors, row-major, exactly the
+ *              `par{2}`/`par{1}` tensors of DistMaterialMatrix
+ *              (Public/Material_Matrix_Generate_Mult_Point.m:28-31,38-48).
+ *   mat_id     [N] host array of material indices per vertex (NULL: all 0);
+ *              the reference's `Standard_inner_idx` is mat_id == 1.
+ * Replaces the per-k operand setup SVD_Matrix_Generate.m:24-30 +
+ * Interpolation_Material_Matrix_Mult_Point_vertex.m:14-27 + Convert2gpuarray.m:2-20:
+ * the device never stores the sparse N_int/M_int/J_int or Pi.Pr/Pi.Qr; it
+ * evaluates them from these tables. */
+int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const double* theta, const double* delta,
+                     int n_mat, const double* mat_table, const unsigned char* mat_id);
+
+/* Change the Bloch phases of all nb k-points (theta [nb][3]); grid and material stay. */
+int fame_plan_set_theta(fame_plan* plan, const double* theta);
+
+int fame_plan_destroy(fame_plan* plan);
+
+/* Wait for all work queued on the plan's stream. */
+int fame_sync(fame_plan* plan);
+
+/* The plan's hipStream_t (as void*), for callers that order their own work after it. */
+void* fame_plan_stream(fame_plan* plan);
+
+/* Run the plan's work on a caller-owned hipStream_t (e.g. the framework's current
+ * stream) instead of its own; NULL switches back to the plan's own stream. */
+int fame_plan_set_stream(fame_plan* plan, void* stream);
+
+/* Measurement aid (no reference counterpart): with enable != 0 every fame_fncB
+ * records HIP events on the plan's stream between its five kernels
+ * (z-pass+Pi mix, xy-pass inverse, material stencil, xy-pass forward,
+ * z-pass+Pi^H mix) and accumulates their durations; each call then waits for
+ * its own completion.  fame_plan_stage_times returns the accumulated
+ * milliseconds per stage (ms[5]) and the number of profiled calls. */
+int fame_plan_profile(fame_plan* plan, int enable);
+int fame_plan_stage_times(fame_plan* plan, double* ms, long long* calls);
+
+/* Psi / Phi ("notranspose", istrans = 0) and Psis / Phis ("transpose", istrans = 1)
+ * on a 3-component field [nb][3N]:  y_c = S .* (ifftn(x_c) * sqrt(N)),
+ * resp. y_c = fftn(conj(S) .* x_c) / sqrt(N).
+ * Replaces subroutines/gMtxVecMul_FFT.m:1-61 (MtxVecMul_FFT.m:1-65) for QP
+ * boundary conditions, where Psi == Phi. */
+int fame_MtxVecMul_FFT(fame_plan* plan, int istrans, const void* in, void* out);
+
+/* fnc.Pr / fnc.Qr : [nb][2N] -> [nb][3N];  fnc.Prs / fnc.Qrs : [nb][3N] -> [nb][2N].
+ * Replace fun_Lanczos/Funhand_Generate_Lanzcos.m:79-82. */
+int fame_Pr(fame_plan* plan, const void* in, void* out);
+int fame_Qr(fame_plan* plan, const void* in, void* out);
+int fame_Prs(fame_plan* plan, const void* in, void* out);
+int fame_Qrs(fame_plan* plan, const void* in, void* out);
+
+/* fncA: [nb][4N] -> [nb][4N].  Replaces fun_Lanczos/fncA.m:3-4. */
+int fame_fncA(fame_plan* plan, const void* in, void* out);
+
+/* fncB: [nb][4N] -> [nb][4N].  Replaces fun_Lanczos/fncB.m:3-6
+ * (and fun_no_inv/fncB_Generate_no_inv.m:6-8). */
+int fame_fncB(fame_plan* plan, const void* in, void* out);
+
+/* Interpolated material sandwich on twisted real-space fields [nb][6N] = [h (3N); e (3N)]:
+ * out = [M_int h + J_int^H e; J_int h + N_int e]  (fun_Lanczos/fncB.m:4-5). */
+int fame_material_apply(fame_plan* plan, const void* in, void* out);
+
+/* fncinvB: y = B \ x by unpreconditioned CG from x0 = 0 for all nb items in lock
+ * step; stops when ||x - B y|| <= tol ||x|| for every item.  iters[nb] and
+ * relres[nb] (may be NULL) receive per-item iteration counts and the final
+ * recursive relative residuals.  Replaces fun_Lanczos/fncinvB.m:4 (MATLAB pcg).
+ * Returns FAME_ERR_NOT_CONVERGED (result still written) if maxit is reached. */
+int fame_fncinvB(fame_plan* plan, const void* in, void* out, double tol, int maxit, int* iters, double* relres);
+
+/* Thick-restart B-orthogonal Lanczos on B^{-1} A (fun_Lanczos/Lanczos.m:1-156)
+ * with fncA / fncB / fncinvB(cg_tol, cg_maxit) as operators, for all nb
+ * k-points in lock step.
+ *   k        wanted eigenvalues per item, p Krylov dimension (p = 2k in the reference)
+ *   v0       [nb][4N] device start vectors (B-normalised inside, Lanczos.m:8)
+ *   ev       [nb][k][4N] device output (eigenvectors, column-major per item)
+ *   ew       [nb][k] host output: eigenvalues d of B^{-1}A (omega = 1/d)
+ *   iters    [nb] host output: Lanczos steps taken (may be NULL)
+ * Returns FAME_ERR_NOT_CONVERGED when maxiter restarts did not converge all items. */
+int fame_lanczos(fame_plan* plan, int k, double tol, int maxiter, int p, const void* v0, double cg_tol, int cg_maxit,
+                 void* ev, double* ew, int* iters);
+
+/* Eigen_Restoration_Chiral_no_inv (fun_no_inv/Eigen_Restoration_Chiral_no_inv.m:3-25):
+ * for nvec NFSEP vectors per item ([nb][nvec][4N]) returns [nb][nvec][6N] =
+ * [e (3N); h (3N)] with e = -i (N Qr(v_b) + J Pr(v_a)), h = -i (J^H Qr(v_b) + M Pr(v_a)). */
+int fame_eigen_restoration(fame_plan* plan, int nvec, const void* ev, void* e_h);
+
+/* Batched complex dot products out[b] = x[b]^H y[b] over len elements,
+ * written to device memory (nb complex). */
+int fame_dot(fame_plan* plan, const void* x, const void* y, long long len, void* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FAME_ABI_H */

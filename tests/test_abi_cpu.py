@@ -1,0 +1,80 @@
+"""CPU-side checks of the boundary: the library loads and exports every C-ABI
+symbol the header declares; host-side setup (materials, structures) matches the
+oracle's restatement.  No GPU calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "fame_abi.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fame_[a-zA-Z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header_symbols(pkg):
+    lib = pkg.load_library()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(pkg.exported_symbols()) == set(syms)
+    assert b"gfx950" in lib.fame_version()
+
+
+def test_plan_create_rejects_bad_args_without_gpu(pkg):
+    """Argument validation happens before any device call."""
+    lib = pkg.load_library()
+    h = ctypes.c_void_p()
+    th = (ctypes.c_double * 3)(0.1, 0.2, 0.3)
+    de = (ctypes.c_double * 3)(1, 1, 1)
+    tab = (ctypes.c_double * 54)()
+    assert lib.fame_plan_create(ctypes.byref(h), 0, 4, 4, 1, th, de, 1, tab, None) == -1
+    assert lib.fame_plan_create(ctypes.byref(h), 17, 4, 4, 1, th, de, 1, tab, None) == -3
+    assert b"prime factor" in lib.fame_last_error()
+    assert lib.fame_plan_create(ctypes.byref(h), 4, 4, 4, 1, th, de, 0, tab, None) == -1
+
+
+def test_material_slots_match_oracle(pkg, fo):
+    S = pkg.structures
+    for A in (S.SC_LATTICE, S.FCC_LATTICE):
+        t = S.bi_isotropic_table(13.0, 0.2, A)
+        n, m, j = fo.material_tensors(13 * np.eye(3), np.eye(3), 0.2j * np.eye(3), -0.2j * np.eye(3), A)
+        assert np.allclose(t[1, 0], n) and np.allclose(t[1, 1], m) and np.allclose(t[1, 2], j)
+        assert np.allclose(t[0, 0], A.T @ A)
+
+
+def test_woodpile_geometry(pkg):
+    S = pkg.structures
+    ids = S.woodpile_sc_ids((24, 24, 24))
+    # each layer: rows i with |i/24 - centre| < 0.125 (open) are inside: i = 4..8 -> 5 of 24
+    assert ids.mean() == pytest.approx(5 / 24)
+    c = ids.reshape(24, 24, 24)
+    assert c[0, 6, :].all() and not c[0, 12, :].any()   # layer 1: along x near y = 1/4
+    assert c[6, :, 6].all() and not c[6, :, 12].any()   # layer 2: along y near x = 1/4
+    assert c[12, 18, :].all()                            # layer 3: near y = 3/4
+    assert c[18, :, 18].all()                            # layer 4: near x = 3/4
+
+
+def test_sc_band_path(pkg):
+    S = pkg.structures
+    ks = S.sc_band_path()
+    assert ks.shape == (40, 3)
+    assert np.allclose(ks[0], 0) and np.allclose(ks[10], [np.pi, 0, 0]) and np.allclose(ks[20], [np.pi, np.pi, 0])
+    assert np.allclose(ks[30], [np.pi] * 3)
+
+
+def test_sphere_and_tetra_fill(pkg):
+    S = pkg.structures
+    A = S.FCC_LATTICE
+    vol = abs(np.linalg.det(A))
+    ids = S.sphere_ids((48, 48, 48), 0.12, A)
+    assert abs(ids.mean() - 4 / 3 * np.pi * 0.12 ** 3 / vol) < 0.1 * 4 / 3 * np.pi * 0.12 ** 3 / vol
+    t = S.tetra_ids((48, 48, 48), 0.3, A)
+    tv = 0.3 ** 3 / (6 * np.sqrt(2))
+    assert abs(t.mean() - tv / vol) < 0.3 * tv / vol  # closed faces: boundary vertices count at n=48

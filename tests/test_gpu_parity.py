@@ -1,0 +1,279 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Tolerance: matvec outputs within 1e-12 relative (FP64), eigenvalues within
+1e-10 relative, residual ||A x - d B x|| / ||x|| <= 1e-8.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MATVEC_RTOL = 1e-12
+rng = np.random.default_rng(2024)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def random_tables(n_mat, hermitian=True):
+    t = crand(n_mat, 3, 3, 3)
+    if hermitian:
+        t[:, :2] = t[:, :2] + np.conj(np.transpose(t[:, :2], (0, 1, 3, 2))) + 6 * np.eye(3)
+    return t
+
+
+def oracle_problem(fo, dims, theta, table, ids):
+    T = table[ids]
+    return fo.Problem(dims, theta, T[:, 0], T[:, 1], T[:, 2])
+
+
+def woodpile(pkg, dims, gamma=0.2):
+    S = pkg.structures
+    return S.bi_isotropic_table(13.0, gamma, S.SC_LATTICE), S.woodpile_sc_ids(dims)
+
+
+DIMS = [(4, 5, 6), (8, 8, 8), (12, 10, 6), (7, 3, 9), (24, 24, 24)]
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_MtxVecMul_FFT(pkg, fo, torch_mod, dims):
+    say(f"MtxVecMul_FFT dims={dims}")
+    thetas = np.array([[0.4, -1.3, 2.2], [1.5, 0.1, -0.7]])
+    table, ids = random_tables(1), np.zeros(np.prod(dims), np.uint8)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    N = plan.N
+    x = crand(2, 3 * N)
+    yn = plan.MtxVecMul_FFT(torch_mod.as_tensor(x), "notranspose").cpu().numpy()
+    yt = plan.MtxVecMul_FFT(torch_mod.as_tensor(x), "transpose").cpu().numpy()
+    for b in range(2):
+        A = fo.svd_matrix_generate(dims, thetas[b], 1.0 / np.array(dims))
+        assert rel(yn[b], fo.MtxVecMul_FFT(x[b], dims, "notranspose", A["S"])) < MATVEC_RTOL
+        assert rel(yt[b], fo.MtxVecMul_FFT(x[b], dims, "transpose", A["S"])) < MATVEC_RTOL
+    plan.close()
+
+
+@pytest.mark.parametrize("dims", [(4, 5, 6), (8, 8, 8), (6, 10, 12)])
+def test_Pr_Qr_Prs_Qrs(pkg, fo, torch_mod, dims):
+    say(f"Pr/Qr/Prs/Qrs dims={dims}")
+    theta = np.array([0.9, -0.4, 1.7])
+    plan = pkg.Plan(dims, theta, random_tables(1), None)
+    N = plan.N
+    A = fo.svd_matrix_generate(dims, theta, 1.0 / np.array(dims))
+    f = fo.Fnc(A)
+    x2, x3 = crand(2 * N), crand(3 * N)
+    T = torch_mod.as_tensor
+    assert rel(plan.Pr(T(x2)).cpu().numpy(), f.Pr(x2)) < MATVEC_RTOL
+    assert rel(plan.Qr(T(x2)).cpu().numpy(), f.Qr(x2)) < MATVEC_RTOL
+    assert rel(plan.Prs(T(x3)).cpu().numpy(), f.Prs(x3)) < MATVEC_RTOL
+    assert rel(plan.Qrs(T(x3)).cpu().numpy(), f.Qrs(x3)) < MATVEC_RTOL
+    plan.close()
+
+
+def test_fncA(pkg, fo, torch_mod):
+    say("fncA")
+    dims = (6, 5, 4)
+    theta = np.array([0.3, 0.2, 0.1])
+    plan = pkg.Plan(dims, theta, random_tables(1), None)
+    A = fo.svd_matrix_generate(dims, theta, 1.0 / np.array(dims))
+    x = crand(4 * plan.N)
+    assert rel(plan.fncA(torch_mod.as_tensor(x)).cpu().numpy(), fo.fncA(x, A["Sigma_r"])) < MATVEC_RTOL
+    plan.close()
+
+
+@pytest.mark.parametrize("dims,n_mat,herm", [((5, 4, 6), 3, True), ((8, 8, 8), 4, False), ((10, 9, 7), 2, True),
+                                              ((3, 3, 3), 2, True)])
+def test_material_apply(pkg, fo, torch_mod, dims, n_mat, herm):
+    say(f"material_apply dims={dims} n_mat={n_mat} herm={herm}")
+    theta = np.array([0.7, -2.5, 1.3])
+    table = random_tables(n_mat, herm)
+    ids = rng.integers(0, n_mat, np.prod(dims)).astype(np.uint8)
+    plan = pkg.Plan(dims, theta, table, ids)
+    P = oracle_problem(fo, dims, theta, table, ids)
+    N = plan.N
+    h, e = crand(3 * N), crand(3 * N)
+    y = plan.material_apply(torch_mod.as_tensor(np.concatenate([h, e]))).cpu().numpy()
+    B = P.mtx_B
+    yh = B["M_int"] @ h + B["J_int"].conj().T @ e
+    ye = B["J_int"] @ h + B["N_int"] @ e
+    assert rel(y, np.concatenate([yh, ye])) < MATVEC_RTOL
+    plan.close()
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8), (6, 10, 12), (12, 12, 12), (24, 24, 24)])
+def test_fncB_woodpile(pkg, fo, torch_mod, dims):
+    say(f"fncB woodpile dims={dims}")
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE) for k in
+                       ([0.5, 0.3, 0.1], [0.0, 0.0, 0.0], [1.0, 1.0, 0.0])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    x = crand(3, 4 * plan.N)
+    y = plan.fncB(torch_mod.as_tensor(x)).cpu().numpy()
+    for b in range(3):
+        ref = oracle_problem(fo, dims, thetas[b], table, ids).fncB(x[b])
+        assert rel(y[b], ref) < MATVEC_RTOL
+    plan.close()
+
+
+def test_fncB_random_materials_fcc(pkg, fo, torch_mod):
+    say("fncB random materials, FCC covariant tables")
+    S = pkg.structures
+    dims = (6, 8, 10)
+    theta = S.theta_of_k(np.array([1.1, -0.3, 2.0]), S.FCC_LATTICE)
+    table = np.stack([S.material_slots(np.eye(3) * e, np.eye(3), 1j * g * np.eye(3), -1j * g * np.eye(3),
+                                       S.FCC_LATTICE) for e, g in ((1, 0), (13, 0.5), (5, 1.2))])
+    ids = rng.integers(0, 3, np.prod(dims)).astype(np.uint8)
+    plan = pkg.Plan(dims, theta, table, ids)
+    x = crand(4 * plan.N)
+    ref = oracle_problem(fo, dims, theta, table, ids).fncB(x)
+    assert rel(plan.fncB(torch_mod.as_tensor(x)).cpu().numpy(), ref) < MATVEC_RTOL
+    plan.close()
+
+
+def test_batched_equals_plain(pkg, torch_mod):
+    say("batched fncB == per-k plain fncB")
+    S = pkg.structures
+    dims = (12, 12, 12)
+    ks = S.sc_band_path()[::7]
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in ks])
+    table, ids = woodpile(pkg, dims)
+    x = crand(len(ks), 4 * np.prod(dims))
+    pb = pkg.Plan(dims, thetas, table, ids)
+    yb = pb.fncB(torch_mod.as_tensor(x)).cpu().numpy()
+    for b in range(len(ks)):
+        p1 = pkg.Plan(dims, thetas[b], table, ids)
+        y1 = p1.fncB(torch_mod.as_tensor(x[b])).cpu().numpy()
+        assert np.array_equal(y1, yb[b])
+        p1.close()
+    pb.close()
+
+
+def test_fncinvB(pkg, fo, torch_mod):
+    say("fncinvB (CG)")
+    dims = (8, 8, 8)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE) for k in ([0.5, 0.3, 0.1], [0.2, 0.0, 0.9])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    x = crand(2, 4 * plan.N)
+    y, iters, relres = plan.fncinvB(torch_mod.as_tensor(x), tol=1e-12, maxit=2000)
+    y = y.cpu().numpy()
+    for b in range(2):
+        P = oracle_problem(fo, dims, thetas[b], table, ids)
+        assert np.linalg.norm(P.fncB(y[b]) - x[b]) <= 2e-12 * np.linalg.norm(x[b])
+        yo, flag, _, _ = fo.pcg(P.fncB, x[b], 1e-12, 2000)
+        assert rel(y[b], yo) < 1e-10
+        assert relres[b] <= 1e-12 and iters[b] > 0
+    plan.close()
+
+
+def test_fncinvB_maxit_reports_not_converged(pkg, torch_mod):
+    say("fncinvB maxit")
+    dims = (8, 8, 8)
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, np.array([0.3, 0.2, 0.1]), table, ids)
+    x = crand(4 * plan.N)
+    with pytest.raises(pkg.NotConverged):
+        plan.fncinvB(torch_mod.as_tensor(x), tol=1e-12, maxit=3)
+    plan.close()
+
+
+def test_lanczos_vacuum_closed_form(pkg, torch_mod):
+    say("Lanczos vacuum")
+    dims = (6, 6, 6)
+    theta = np.pi * np.array([0.5, 0.3, 0.1])
+    S = pkg.structures
+    table = S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE)
+    plan = pkg.Plan(dims, theta, table, np.zeros(216, np.uint8))
+    k = 4
+    ev, ew, it = plan.Lanczos(k, tol=1e-10, maxiter=100)
+    omega = np.sort(1.0 / ew[0])
+    lam = [np.abs((np.exp(1j * (2 * np.pi * np.arange(n) + theta[d]) / n) - 1) * n) ** 2 for d, n in enumerate(dims)]
+    sig = np.sort(np.sqrt(lam[0][None, None, :] + lam[1][None, :, None] + lam[2][:, None, None]).ravel())
+    distinct = sig[np.concatenate([[True], np.diff(sig) > 1e-9 * sig[-1]])]
+    assert np.allclose(omega, distinct[:k], rtol=1e-10)
+    plan.close()
+
+
+def test_lanczos_woodpile_matches_oracle(pkg, fo, torch_mod):
+    say("Lanczos woodpile n=6 vs oracle Lanczos and dense GEP")
+    import scipy.linalg
+
+    dims = (6, 6, 6)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE) for k in ([0.5, 0.3, 0.1], [0.1, 0.7, 0.2])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    k = 5
+    ev, ew, it = plan.Lanczos(k, tol=1e-10, maxiter=200)
+    evn = ev.cpu().numpy()
+    for b in range(2):
+        P = oracle_problem(fo, dims, thetas[b], table, ids)
+        n4 = 4 * P.N
+        Ad = np.stack([P.fncA(e) for e in np.eye(n4, dtype=complex)], 1)
+        Bd = np.stack([P.fncB(e) for e in np.eye(n4, dtype=complex)], 1)
+        d = np.sort(scipy.linalg.eigh(Ad, Bd, eigvals_only=True))[::-1][:k]
+        assert np.allclose(np.sort(ew[b])[::-1], d, rtol=1e-10)
+        for j in range(k):
+            x = evn[b, j]
+            r = P.fncA(x) - ew[b, j] * P.fncB(x)
+            assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(x) * abs(ew[b, j])
+    plan.close()
+
+
+def test_eigen_restoration(pkg, fo, torch_mod):
+    say("Eigen_Restoration_Chiral_no_inv")
+    dims = (6, 5, 4)
+    theta = np.array([0.6, -0.2, 1.1])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, theta, table, ids)
+    P = oracle_problem(fo, dims, theta, table, ids)
+    ev = crand(2, 4 * plan.N)
+    out = plan.Eigen_Restoration_Chiral_no_inv(torch_mod.as_tensor(ev[None])).cpu().numpy()[0]
+    e, h = fo.eigen_restoration(ev.T, P.fnc, P.mtx_B)
+    for j in range(2):
+        assert rel(out[j], np.concatenate([e[:, j], h[:, j]])) < MATVEC_RTOL
+    plan.close()
+
+
+def test_n60_invariants(pkg, torch_mod):
+    """Full-size (n = 60) size-independent properties: vacuum fncB = I, Hermiticity,
+    linearity, and Prs(Pr x) = x."""
+    say("n=60 invariants")
+    S = pkg.structures
+    dims = (60, 60, 60)
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    table, ids = woodpile(pkg, dims)
+    T = torch_mod.as_tensor
+    pv = pkg.Plan(dims, theta, table, np.zeros(np.prod(dims), np.uint8))
+    x, y = crand(4 * pv.N), crand(4 * pv.N)
+    assert rel(pv.fncB(T(x)).cpu().numpy(), x) < 1e-13
+    x2 = crand(2 * pv.N)
+    assert rel(pv.Prs(pv.Pr(T(x2))).cpu().numpy(), x2) < 1e-13
+    pv.close()
+    pw = pkg.Plan(dims, theta, table, ids)
+    Bx, By = pw.fncB(T(x)).cpu().numpy(), pw.fncB(T(y)).cpu().numpy()
+    a, b = np.vdot(y, Bx), np.conj(np.vdot(x, By))
+    assert abs(a - b) <= 1e-12 * abs(a)
+    assert np.vdot(x, Bx).real > 0
+    Bxy = pw.fncB(T(x + 2.5j * y)).cpu().numpy()
+    assert rel(Bxy, Bx + 2.5j * By) < 1e-13
+    pw.close()
