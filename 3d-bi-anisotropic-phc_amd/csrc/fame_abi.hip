@@ -484,7 +484,7 @@ void* fame_plan_stream(fame_plan* P) { return P ? (void*)P->stream : nullptr; }
 int fame_plan_set_stream(fame_plan* P, void* stream) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   HIP_TRY(hipStreamSynchronize(P->stream));
-  P->stream = stream ? (hipStream_t)stream : P->own_stream;
+  P->stream = (hipStream_t)stream;
   return FAME_OK;
 }
 

@@ -71,8 +71,10 @@ int fame_sync(fame_plan* plan);
 /* The plan's hipStream_t (as void*), for callers that order their own work after it. */
 void* fame_plan_stream(fame_plan* plan);
 
-/* Run the plan's work on a caller-owned hipStream_t (e.g. the framework's current
- * stream) instead of its own; NULL switches back to the plan's own stream. */
+/* Run the plan's work on the given hipStream_t from now on (e.g. the framework's
+ * current stream); NULL is the device's default (null) stream.  The plan's own
+ * stream is the value fame_plan_stream() returns right after creation.  Work
+ * already queued on the previous stream is waited for first. */
 int fame_plan_set_stream(fame_plan* plan, void* stream);
 
 /* Measurement aid (no reference counterpart): with enable != 0 every fame_fncB
