@@ -13,6 +13,7 @@
 
 #include "../../include/fame_abi.h"
 #include "fame_kernels.hpp"
+#include "fame_fft_fast.hpp"
 
 using namespace fame;
 
@@ -91,6 +92,7 @@ struct fame_plan {
   double2* d_q = nullptr;
   CgState* d_cg = nullptr;
   int tab_len = 0;            // n1+n2+n3
+  bool fast = true;           // use the specialised FFT kernels where a size has them
   // per-stage timing of fncB (HIP events between the five kernels), off by default
   bool prof = false;
   hipEvent_t ev[6] = {};
@@ -182,12 +184,113 @@ unsigned grid1d(long long len) {
   return (unsigned)g;
 }
 
+// ---- specialised sizes ----------------------------------------------------
+// (N, A, B) with N = A*B: the four-step split used by k_xyfast / k_zfast.
+#define FAME_FAST_SIZES(X) \
+  X(8, 2, 4) X(12, 3, 4) X(16, 4, 4) X(20, 4, 5) X(24, 4, 6) X(32, 4, 8) X(40, 5, 8) X(48, 6, 8) \
+  X(60, 6, 10) X(64, 8, 8) X(72, 8, 9) X(80, 8, 10) X(90, 9, 10) X(96, 8, 12)
+
+constexpr int kZTP = 16;  // pencils per z work-group in the specialised pass
+
+template <int N>
+constexpr int xy_threads() { return N >= 72 ? 512 : 256; }
+
+bool fast_size(int n) {
+#define X(N, A, B) if (n == N) return true;
+  FAME_FAST_SIZES(X)
+#undef X
+  return false;
+}
+
+bool use_fast_xy(const fame_plan* P) { return P->n1 == P->n2 && fast_size(P->n1); }
+bool use_fast_z(const fame_plan* P) { return fast_size(P->n3); }
+size_t xyfast_lds(int n) { return (size_t)(n * n + n) * sizeof(double2); }
+size_t zfast_lds(int n3) { return (size_t)(3 * n3 * kZTP + n3) * sizeof(double2); }
+
+template <int N, int A, int B>
+int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs,
+                   int nf, double sc) {
+  constexpr int NT = xy_threads<N>();
+  const dim3 grid(P->n3, nf, P->nb);
+  const size_t lds = xyfast_lds(N);
+  if (dir > 0) {
+    if (twist == 1)
+      hipLaunchKernelGGL((k_xyfast<N, A, B, 1, 1, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_tws, P->tab_len, P->d_tw, sc);
+    else
+      hipLaunchKernelGGL((k_xyfast<N, A, B, 1, 0, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_tws, P->tab_len, P->d_tw, sc);
+  } else {
+    if (twist == 2)
+      hipLaunchKernelGGL((k_xyfast<N, A, B, -1, 2, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_tws, P->tab_len, P->d_tw, sc);
+    else
+      hipLaunchKernelGGL((k_xyfast<N, A, B, -1, 0, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
+                         P->d_tws, P->tab_len, P->d_tw, sc);
+  }
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+template <int N, int A, int B>
+int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs,
+                  int nslot, int grp_base, double sc) {
+  constexpr int NT = 256;
+  const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->nb);
+  const size_t lds = zfast_lds(N);
+  if (dir > 0) {
+    if (mode == ZM_MIX_IN)
+      hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
+                         out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+    else
+      hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
+                         out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+  } else {
+    if (mode == ZM_MIX_OUT)
+      hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in,
+                         ibs, out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+    else
+      hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
+                         out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+  }
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
+template <int N, int A, int B>
+int fast_set_attrs() {
+  constexpr int NT = xy_threads<N>();
+  const int lx = (int)xyfast_lds(N), lz = (int)zfast_lds(N);
+  HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, 1, 0, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, 1, 1, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, -1, 0, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, -1, 2, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_PLAIN, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_PLAIN, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  return FAME_OK;
+}
+
+int set_fast_attrs(int n) {
+#define X(N, A, B) if (n == N) return fast_set_attrs<N, A, B>();
+  FAME_FAST_SIZES(X)
+#undef X
+  return FAME_OK;
+}
+
 // ---- launch helpers -------------------------------------------------------
 int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs, int nslot,
           int grp_base) {
+  const double sc = 1.0 / std::sqrt((double)P->n3);
+  if (P->fast && use_fast_z(P)) {
+#define X(N, A, B) \
+    if (P->n3 == N) return z_fast_launch<N, A, B>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc);
+    FAME_FAST_SIZES(X)
+#undef X
+  }
   const dim3 grid((P->geo.n12 + kZTile - 1) / kZTile, nslot, P->nb);
   const size_t lds = zpass_lds(P);
-  const double sc = 1.0 / std::sqrt((double)P->n3);
   if (dir > 0) {
     if (mode == ZM_MIX_IN)
       hipLaunchKernelGGL((k_zpass<1, ZM_MIX_IN>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
@@ -208,9 +311,15 @@ int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, dou
 }
 
 int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs, int nf) {
+  const double sc = 1.0 / std::sqrt((double)P->n1 * (double)P->n2);
+  if (P->fast && use_fast_xy(P)) {
+#define X(N, A, B) \
+    if (P->n1 == N) return xy_fast_launch<N, A, B>(P, dir, twist, in, ibs, out, obs, nf, sc);
+    FAME_FAST_SIZES(X)
+#undef X
+  }
   const dim3 grid(P->n3, nf, P->nb);
   const size_t lds = xypass_lds(P);
-  const double sc = 1.0 / std::sqrt((double)P->n1 * (double)P->n2);
 #define XY_LAUNCH(D, T)                                                                                   \
   hipLaunchKernelGGL((k_xypass<D, T>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs, P->d_tws, \
                      P->tab_len, P->d_tw, P->d_perm, sc)
@@ -444,6 +553,12 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_MIX_OUT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
 #undef PLAN_HIP
+  {
+    int rc = FAME_OK;
+    if (use_fast_xy(P)) rc = set_fast_attrs(n1);
+    if (rc == FAME_OK && use_fast_z(P)) rc = set_fast_attrs(n3);
+    if (rc != FAME_OK) return fail(rc);
+  }
   int rc = upload_k_tables(P);
   if (rc != FAME_OK) return fail(rc);
   *plan = P;
@@ -485,6 +600,12 @@ int fame_plan_set_stream(fame_plan* P, void* stream) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   HIP_TRY(hipStreamSynchronize(P->stream));
   P->stream = (hipStream_t)stream;
+  return FAME_OK;
+}
+
+int fame_plan_set_generic(fame_plan* P, int generic) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  P->fast = (generic == 0);
   return FAME_OK;
 }
 

@@ -1,0 +1,319 @@
+// Compile-time specialised FFT passes for the hot grid sizes.
+//
+// Every line of length N = A*B is transformed by the four-step factorisation
+// (Gentleman & Sande 1966; Bailey 1990):  with x[t + B j] (t < B, j < A),
+//     X[k + A q] = sum_t W_N^{t k} W_B^{t q} sum_j x[t + B j] W_A^{j k}.
+// Phase A: one thread per column t runs the A-point DFT over j in registers,
+// multiplies by W_N^{t k} and writes back to the same LDS positions.  Phase B:
+// one thread per row k runs the B-point DFT over t in registers and emits
+// X[k + A q] directly in natural order.  The A- and B-point DFTs are themselves
+// built recursively from radix-2/3/4/5 kernels in registers (rdft), so every
+// index is a compile-time constant.  Twiddles come from an LDS copy of the
+// line's table W_N^m = e^{-2 pi i m/N} (host-computed in long double).
+#pragma once
+#include "fame_kernels.hpp"
+
+namespace fame {
+
+template <int S>
+constexpr int split_factor() {
+  return (S % 4 == 0 && S > 4) ? 4 : (S % 2 == 0 && S > 2) ? 2 : (S % 3 == 0 && S > 3) ? 3 : (S % 5 == 0 && S > 5) ? 5 : S;
+}
+
+template <int DIR>
+__device__ __forceinline__ double2 twmul(double2 v, double2 w) {
+  return (DIR < 0) ? cmul(v, w) : cmulc(v, w);
+}
+
+// In-register DFT of length S (natural order in and out).  W_S^m = tw[m * ts].
+template <int S, int DIR>
+__device__ __forceinline__ void rdft(double2* v, const double2* __restrict__ tw, int ts) {
+  if constexpr (S == 1) {
+    return;
+  } else if constexpr (S == 2) {
+    dft2<DIR>(v[0], v[1]);
+  } else if constexpr (S == 3) {
+    dft3<DIR>(v[0], v[1], v[2]);
+  } else if constexpr (S == 4) {
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (S == 5) {
+    dft5<DIR>(v[0], v[1], v[2], v[3], v[4]);
+  } else {
+    constexpr int P = split_factor<S>();
+    constexpr int Q = S / P;
+    static_assert(P > 1 && Q > 1, "rdft: S must factor into 2,3,4,5");
+    double2 w[S];
+#pragma unroll
+    for (int t = 0; t < Q; ++t) {
+      double2 u[P];
+#pragma unroll
+      for (int j = 0; j < P; ++j) u[j] = v[t + Q * j];
+      rdft<P, DIR>(u, tw, ts * Q);
+#pragma unroll
+      for (int k = 0; k < P; ++k) w[t + Q * k] = (t * k == 0) ? u[k] : twmul<DIR>(u[k], tw[(t * k) * ts]);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      double2 u[Q];
+#pragma unroll
+      for (int t = 0; t < Q; ++t) u[t] = w[t + Q * k];
+      rdft<Q, DIR>(u, tw, ts * P);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) v[k + P * q] = u[q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// xy-plane pass (N1 = N2 = N = A*B): one work-group per (z-plane, component, batch).
+// LDS: the plane (N*N) + the twiddle table (N).
+// ---------------------------------------------------------------------------
+template <int N, int A, int B, int DIR, int TWIST, int NT>
+__global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict__ in, long long in_bstride,
+                                               double2* __restrict__ out, long long out_bstride,
+                                               const double2* __restrict__ tws, int tws_bstride,
+                                               const double2* __restrict__ twg, double scale) {
+  static_assert(A * B == N, "k_xyfast: N = A*B");
+  extern __shared__ double2 sm[];
+  double2* pl = sm;
+  double2* tw = sm + N * N;
+  const int i3 = blockIdx.x, f = blockIdx.y, b = blockIdx.z;
+  const long long base = (long long)f * g.N + (long long)i3 * (N * N);
+  const double2* __restrict__ src = in + b * in_bstride + base;
+  double2* __restrict__ dst = out + b * out_bstride + base;
+  const double2* __restrict__ S = tws + (long long)b * tws_bstride;
+  const double2* __restrict__ twn = twg + g.ax[0].tw_off;
+  for (int t = threadIdx.x; t < N; t += NT) tw[t] = twn[t];
+  double2 sz = make_double2(1, 0);
+  if (TWIST) sz = S[2 * N + i3];
+  __syncthreads();
+
+  // x phase A: item (r, t), t fastest; global -> regs -> LDS (positions t + B k of row r)
+  for (int it = threadIdx.x; it < N * B; it += NT) {
+    const int r = it / B, t = it - (it / B) * B;
+    double2 u[A];
+    double2 syz = make_double2(1, 0);
+    if (TWIST == 2) syz = cmul(S[N + r], sz);
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+      double2 v = src[r * N + t + B * j];
+      if (TWIST == 2) v = cmulc(v, cmul(S[t + B * j], syz));
+      u[j] = v;
+    }
+    rdft<A, DIR>(u, tw, B);
+    pl[r * N + t] = u[0];
+#pragma unroll
+    for (int k = 1; k < A; ++k) pl[r * N + t + B * k] = twmul<DIR>(u[k], tw[t * k]);
+  }
+  __syncthreads();
+
+  // x phase B: item (r, k); read B values, hold, barrier, write natural order k + A q
+  constexpr int IB = (N * A + NT - 1) / NT;
+  double2 ub[IB][B];
+#pragma unroll
+  for (int i = 0; i < IB; ++i) {
+    const int it = threadIdx.x + i * NT;
+    if (it < N * A) {
+      const int r = it / A, k = it - (it / A) * A;
+#pragma unroll
+      for (int t = 0; t < B; ++t) ub[i][t] = pl[r * N + t + B * k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IB; ++i) {
+    const int it = threadIdx.x + i * NT;
+    if (it < N * A) {
+      const int r = it / A, k = it - (it / A) * A;
+      rdft<B, DIR>(ub[i], tw, A);
+#pragma unroll
+      for (int q = 0; q < B; ++q) pl[r * N + k + A * q] = ub[i][q];
+    }
+  }
+  __syncthreads();
+
+  // y phase A: item (c, t), c fastest; rows t + B j of column c, in place
+  for (int it = threadIdx.x; it < N * B; it += NT) {
+    const int t = it / N, c = it - (it / N) * N;
+    double2 u[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) u[j] = pl[(t + B * j) * N + c];
+    rdft<A, DIR>(u, tw, B);
+    pl[t * N + c] = u[0];
+#pragma unroll
+    for (int k = 1; k < A; ++k) pl[(t + B * k) * N + c] = twmul<DIR>(u[k], tw[t * k]);
+  }
+  __syncthreads();
+
+  // y phase B: item (c, k), c fastest; rows B k + t -> output rows k + A q in HBM
+  for (int it = threadIdx.x; it < N * A; it += NT) {
+    const int k = it / N, c = it - (it / N) * N;
+    double2 u[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) u[t] = pl[(B * k + t) * N + c];
+    rdft<B, DIR>(u, tw, A);
+    double2 sxz = make_double2(1, 0);
+    if (TWIST == 1) sxz = cmul(S[c], sz);
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int row = k + A * q;
+      double2 v = cscale(u[q], scale);
+      if (TWIST == 1) v = cmul(v, cmul(S[N + row], sxz));
+      dst[row * N + c] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z-pencil pass (n3 = N3 = A*B), TP consecutive pencils per work-group.
+// LDS: [3][N3][TP] (pencil index fastest) + twiddles (N3).
+// ---------------------------------------------------------------------------
+template <int N3, int A, int B, int DIR, int MODE, int TP, int NT>
+__global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__ in, long long in_bstride,
+                                              double2* __restrict__ out, long long out_bstride,
+                                              const double2* __restrict__ lam, int lam_bstride,
+                                              const double2* __restrict__ twg, double scale, int grp_base) {
+  static_assert(A * B == N3, "k_zfast: N3 = A*B");
+  extern __shared__ double2 sm[];
+  double2* tw = sm + 3 * N3 * TP;
+  const int n12 = g.n12;
+  const long long N = g.N;
+  const int q0 = blockIdx.x * TP;
+  const int slot = blockIdx.y;
+  const int grp = grp_base + slot;
+  const int b = blockIdx.z;
+  const double2* __restrict__ lb = lam + (long long)b * lam_bstride;
+  const double2* __restrict__ twn = twg + g.ax[2].tw_off;
+  for (int t = threadIdx.x; t < N3; t += NT) tw[t] = twn[t];
+  auto L = [&](int c, int z, int pp) -> double2& { return sm[(c * N3 + z) * TP + pp]; };
+
+  if (MODE == ZM_MIX_IN) {
+    // phase 0: load + Pi mix in natural order
+    const double2* __restrict__ xin = in + b * in_bstride + (long long)(2 * slot) * N;
+    for (int it = threadIdx.x; it < N3 * TP; it += NT) {
+      const int k3 = it / TP, pp = it - (it / TP) * TP;
+      const int q = q0 + pp;
+      double2 v0 = make_double2(0, 0), v1 = v0, v2 = v0;
+      if (q < n12) {
+        const long long P = q + (long long)n12 * k3;
+        const double2 u1 = xin[P], u2 = xin[N + P];
+        const int k2 = q / g.n1, k1 = q - k2 * g.n1;
+        double2 p1[3], p2[3];
+        pi_vectors(lb[k1], lb[g.n1 + k2], lb[g.n1 + g.n2 + k3], p1, p2);
+        if (grp == 0) {
+          v0 = cfmac(cscale(cmulc(u1, p2[0]), -1.0), p1[0], u2);
+          v1 = cfmac(cscale(cmulc(u1, p2[1]), -1.0), p1[1], u2);
+          v2 = cfmac(cscale(cmulc(u1, p2[2]), -1.0), p1[2], u2);
+        } else {
+          v0 = cfma(cmul(p1[0], u1), p2[0], u2);
+          v1 = cfma(cmul(p1[1], u1), p2[1], u2);
+          v2 = cfma(cmul(p1[2], u1), p2[2], u2);
+        }
+      }
+      L(0, k3, pp) = v0;
+      L(1, k3, pp) = v1;
+      L(2, k3, pp) = v2;
+    }
+    __syncthreads();
+    // phase A in place: item (c, t, pp)
+    for (int it = threadIdx.x; it < 3 * B * TP; it += NT) {
+      const int pp = it % TP, r = it / TP;
+      const int t = r % B, c = r / B;
+      double2 u[A];
+#pragma unroll
+      for (int j = 0; j < A; ++j) u[j] = L(c, t + B * j, pp);
+      rdft<A, DIR>(u, tw, B);
+      L(c, t, pp) = u[0];
+#pragma unroll
+      for (int k = 1; k < A; ++k) L(c, t + B * k, pp) = twmul<DIR>(u[k], tw[t * k]);
+    }
+  } else {
+    // phase A from HBM: item (c, t, pp)
+    const double2* __restrict__ fin = in + b * in_bstride + (MODE == ZM_MIX_OUT ? (long long)(3 * slot) * N : 0);
+    __syncthreads();
+    for (int it = threadIdx.x; it < 3 * B * TP; it += NT) {
+      const int pp = it % TP, r = it / TP;
+      const int t = r % B, c = r / B;
+      const int q = q0 + pp;
+      double2 u[A];
+#pragma unroll
+      for (int j = 0; j < A; ++j)
+        u[j] = (q < n12) ? fin[c * N + q + (long long)n12 * (t + B * j)] : make_double2(0, 0);
+      rdft<A, DIR>(u, tw, B);
+      L(c, t, pp) = u[0];
+#pragma unroll
+      for (int k = 1; k < A; ++k) L(c, t + B * k, pp) = twmul<DIR>(u[k], tw[t * k]);
+    }
+  }
+  __syncthreads();
+
+  if (MODE == ZM_MIX_OUT) {
+    // phase B into LDS (hold, barrier, write natural order), then phase C: Pi^H mix per point
+    constexpr int IB = (3 * A * TP + NT - 1) / NT;
+    double2 ub[IB][B];
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      const int it = threadIdx.x + i * NT;
+      if (it < 3 * A * TP) {
+        const int pp = it % TP, r = it / TP;
+        const int k = r % A, c = r / A;
+#pragma unroll
+        for (int t = 0; t < B; ++t) ub[i][t] = L(c, B * k + t, pp);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      const int it = threadIdx.x + i * NT;
+      if (it < 3 * A * TP) {
+        const int pp = it % TP, r = it / TP;
+        const int k = r % A, c = r / A;
+        rdft<B, DIR>(ub[i], tw, A);
+#pragma unroll
+        for (int q = 0; q < B; ++q) L(c, k + A * q, pp) = ub[i][q];
+      }
+    }
+    __syncthreads();
+    double2* __restrict__ yout = out + b * out_bstride + (long long)(2 * slot) * N;
+    for (int it = threadIdx.x; it < N3 * TP; it += NT) {
+      const int k3 = it / TP, pp = it - (it / TP) * TP;
+      const int q = q0 + pp;
+      if (q >= n12) continue;
+      const double2 y0 = cscale(L(0, k3, pp), scale);
+      const double2 y1 = cscale(L(1, k3, pp), scale);
+      const double2 y2 = cscale(L(2, k3, pp), scale);
+      const int k2 = q / g.n1, k1 = q - k2 * g.n1;
+      double2 p1[3], p2[3];
+      pi_vectors(lb[k1], lb[g.n1 + k2], lb[g.n1 + g.n2 + k3], p1, p2);
+      double2 o1, o2;
+      if (grp == 0) {
+        o1 = cscale(cfma(cfma(cmul(p2[0], y0), p2[1], y1), p2[2], y2), -1.0);
+        o2 = cfma(cfma(cmul(p1[0], y0), p1[1], y1), p1[2], y2);
+      } else {
+        o1 = cfmac(cfmac(cmulc(y0, p1[0]), p1[1], y1), p1[2], y2);
+        o2 = cfmac(cfmac(cmulc(y0, p2[0]), p2[1], y1), p2[2], y2);
+      }
+      const long long P = q + (long long)n12 * k3;
+      yout[P] = o1;
+      yout[N + P] = o2;
+    }
+  } else {
+    // phase B straight to HBM: item (c, k, pp); output z = k + A q
+    double2* __restrict__ fo = out + b * out_bstride + (MODE == ZM_MIX_IN ? (long long)(3 * slot) * N : 0);
+    for (int it = threadIdx.x; it < 3 * A * TP; it += NT) {
+      const int pp = it % TP, r = it / TP;
+      const int k = r % A, c = r / A;
+      const int q = q0 + pp;
+      double2 u[B];
+#pragma unroll
+      for (int t = 0; t < B; ++t) u[t] = L(c, B * k + t, pp);
+      rdft<B, DIR>(u, tw, A);
+      if (q < n12) {
+#pragma unroll
+        for (int qq = 0; qq < B; ++qq) fo[c * N + q + (long long)n12 * (k + A * qq)] = cscale(u[qq], scale);
+      }
+    }
+  }
+}
+
+}  // namespace fame

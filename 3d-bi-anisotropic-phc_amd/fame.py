@@ -80,6 +80,7 @@ def load_library(path: str = LIB_PATH):
         "fame_eigen_restoration": ([vp, i, vp, vp], i),
         "fame_dot": ([vp, vp, vp, ctypes.c_longlong, vp], i),
         "fame_plan_profile": ([vp, i], i),
+        "fame_plan_set_generic": ([vp, i], i),
         "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
     }
     for name, (args, res) in sig.items():
@@ -97,6 +98,7 @@ def exported_symbols():
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_plan_profile", "fame_plan_stage_times",
+        "fame_plan_set_generic",
     ]
 
 
@@ -181,6 +183,10 @@ class Plan:
 
     def sync(self):
         _check(self.lib.fame_sync(self.h))
+
+    def set_generic(self, generic=True):
+        """Force the runtime-radix FFT kernels (True) or allow the specialised ones (False)."""
+        _check(self.lib.fame_plan_set_generic(self.h, int(bool(generic))))
 
     STAGES = ("zpass_mix_in", "xypass_inv", "material", "xypass_fwd", "zpass_mix_out")
 

@@ -77,6 +77,12 @@ void* fame_plan_stream(fame_plan* plan);
  * already queued on the previous stream is waited for first. */
 int fame_plan_set_stream(fame_plan* plan, void* stream);
 
+/* Kernel selection (no reference counterpart): grid lengths with a compile-time
+ * specialised FFT (8,12,16,20,24,32,40,48,60,64,72,80,90,96; xy needs n1 == n2)
+ * use it by default; generic != 0 forces the runtime-radix kernels that cover
+ * every supported size (used to cross-check the two). */
+int fame_plan_set_generic(fame_plan* plan, int generic);
+
 /* Measurement aid (no reference counterpart): with enable != 0 every fame_fncB
  * records HIP events on the plan's stream between its five kernels
  * (z-pass+Pi mix, xy-pass inverse, material stencil, xy-pass forward,

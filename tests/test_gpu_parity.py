@@ -277,3 +277,25 @@ def test_n60_invariants(pkg, torch_mod):
     Bxy = pw.fncB(T(x + 2.5j * y)).cpu().numpy()
     assert rel(Bxy, Bx + 2.5j * By) < 1e-13
     pw.close()
+
+
+@pytest.mark.parametrize("n", [8, 12, 16, 20, 24, 32, 40, 48, 60, 64, 72, 80, 90, 96])
+def test_specialised_fft_equals_generic(pkg, torch_mod, n):
+    """Compile-time specialised FFT passes == runtime-radix passes (fncB, Psi, Psis, Pr, Qrs)."""
+    say(f"specialised vs generic n={n}")
+    S = pkg.structures
+    dims = (n, n, n)
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, theta, table, ids)
+    N = plan.N
+    T = torch_mod.as_tensor
+    x4, x3, x2 = T(crand(4 * N)), T(crand(3 * N)), T(crand(2 * N))
+    fast = [plan.fncB(x4), plan.MtxVecMul_FFT(x3, "notranspose"), plan.MtxVecMul_FFT(x3, "transpose"),
+            plan.Pr(x2), plan.Qrs(x3)]
+    plan.set_generic(True)
+    gen = [plan.fncB(x4), plan.MtxVecMul_FFT(x3, "notranspose"), plan.MtxVecMul_FFT(x3, "transpose"),
+           plan.Pr(x2), plan.Qrs(x3)]
+    for a, b in zip(fast, gen):
+        assert rel(a.cpu().numpy(), b.cpu().numpy()) < 1e-13
+    plan.close()
