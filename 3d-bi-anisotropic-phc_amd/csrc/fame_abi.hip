@@ -93,6 +93,9 @@ struct fame_plan {
   CgState* d_cg = nullptr;
   int tab_len = 0;            // n1+n2+n3
   bool fast = true;           // use the specialised FFT kernels where a size has them
+  // batch window used by the launch helpers: items [cb0, cb0 + cnb)
+  int cb0 = 0, cnb = 0;
+  int chunk = 0;              // fncB processes the batch in chunks of this many items (0: all)
   // per-stage timing of fncB (HIP events between the five kernels), off by default
   bool prof = false;
   hipEvent_t ev[6] = {};
@@ -101,6 +104,10 @@ struct fame_plan {
 };
 
 namespace {
+
+inline const double2* wlam(const fame_plan* P) { return P->d_lam + (size_t)P->cb0 * P->tab_len; }
+inline const double2* wtws(const fame_plan* P) { return P->d_tws + (size_t)P->cb0 * P->tab_len; }
+inline const double2* wom(const fame_plan* P) { return P->d_omega + (size_t)P->cb0 * 3; }
 
 int build_axis(int n, std::vector<double2>& tw, std::vector<int>& perm, AxisPlan& ap) {
   std::vector<int> r;
@@ -211,22 +218,22 @@ template <int N, int A, int B>
 int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs,
                    int nf, double sc) {
   constexpr int NT = xy_threads<N>();
-  const dim3 grid(P->n3, nf, P->nb);
+  const dim3 grid(P->n3, nf, P->cnb);
   const size_t lds = xyfast_lds(N);
   if (dir > 0) {
     if (twist == 1)
       hipLaunchKernelGGL((k_xyfast<N, A, B, 1, 1, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_tws, P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc);
     else
       hipLaunchKernelGGL((k_xyfast<N, A, B, 1, 0, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_tws, P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc);
   } else {
     if (twist == 2)
       hipLaunchKernelGGL((k_xyfast<N, A, B, -1, 2, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_tws, P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc);
     else
       hipLaunchKernelGGL((k_xyfast<N, A, B, -1, 0, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_tws, P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc);
   }
   CHECK_LAUNCH();
   return FAME_OK;
@@ -236,22 +243,22 @@ template <int N, int A, int B>
 int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs,
                   int nslot, int grp_base, double sc) {
   constexpr int NT = 256;
-  const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->nb);
+  const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->cnb);
   const size_t lds = zfast_lds(N);
   if (dir > 0) {
     if (mode == ZM_MIX_IN)
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
-                         out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
     else
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
-                         out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
   } else {
     if (mode == ZM_MIX_OUT)
       hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in,
-                         ibs, out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+                         ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
     else
       hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
-                         out, obs, P->d_lam, P->tab_len, P->d_tw, sc, grp_base);
+                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
   }
   CHECK_LAUNCH();
   return FAME_OK;
@@ -289,22 +296,22 @@ int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, dou
     FAME_FAST_SIZES(X)
 #undef X
   }
-  const dim3 grid((P->geo.n12 + kZTile - 1) / kZTile, nslot, P->nb);
+  const dim3 grid((P->geo.n12 + kZTile - 1) / kZTile, nslot, P->cnb);
   const size_t lds = zpass_lds(P);
   if (dir > 0) {
     if (mode == ZM_MIX_IN)
       hipLaunchKernelGGL((k_zpass<1, ZM_MIX_IN>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+                         wlam(P), P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
     else
       hipLaunchKernelGGL((k_zpass<1, ZM_PLAIN>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+                         wlam(P), P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
   } else {
     if (mode == ZM_MIX_OUT)
       hipLaunchKernelGGL((k_zpass<-1, ZM_MIX_OUT>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+                         wlam(P), P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
     else
       hipLaunchKernelGGL((k_zpass<-1, ZM_PLAIN>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs,
-                         P->d_lam, P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
+                         wlam(P), P->tab_len, P->d_tw, P->d_perm, sc, grp_base);
   }
   CHECK_LAUNCH();
   return FAME_OK;
@@ -318,10 +325,10 @@ int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, d
     FAME_FAST_SIZES(X)
 #undef X
   }
-  const dim3 grid(P->n3, nf, P->nb);
+  const dim3 grid(P->n3, nf, P->cnb);
   const size_t lds = xypass_lds(P);
 #define XY_LAUNCH(D, T)                                                                                   \
-  hipLaunchKernelGGL((k_xypass<D, T>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs, P->d_tws, \
+  hipLaunchKernelGGL((k_xypass<D, T>), grid, dim3(kBlk), lds, P->stream, P->geo, in, ibs, out, obs, wtws(P), \
                      P->tab_len, P->d_tw, P->d_perm, sc)
   if (dir > 0) {
     if (twist == 1) XY_LAUNCH(1, 1); else XY_LAUNCH(1, 0);
@@ -336,21 +343,30 @@ int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, d
 int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs) {
   const int tx = (P->n1 + kMatTX - 1) / kMatTX, ty = (P->n2 + kMatTY - 1) / kMatTY;
   const int tz = (P->n3 + kMatTZ - 1) / kMatTZ;
-  hipLaunchKernelGGL(k_material, dim3(tx * ty, tz, P->nb), dim3(kBlk), 0, P->stream, P->geo, in, ibs, out, obs,
-                     P->d_mat_id, P->d_mat, P->n_mat, P->d_omega);
+  hipLaunchKernelGGL(k_material, dim3(tx * ty, tz, P->cnb), dim3(kBlk), 0, P->stream, P->geo, in, ibs, out, obs,
+                     P->d_mat_id, P->d_mat, P->n_mat, wom(P));
   CHECK_LAUNCH();
   return FAME_OK;
 }
 
 int twist(fame_plan* P, const double2* in, double2* out, int ncomp, int conj_twist) {
-  hipLaunchKernelGGL(k_twist, dim3(grid1d(P->N), P->nb), dim3(kBlk), 0, P->stream, P->geo, in, out, ncomp,
-                     P->d_tws, P->tab_len, conj_twist);
+  hipLaunchKernelGGL(k_twist, dim3(grid1d(P->N), P->cnb), dim3(kBlk), 0, P->stream, P->geo, in, out, ncomp,
+                     wtws(P), P->tab_len, conj_twist);
   CHECK_LAUNCH();
   return FAME_OK;
 }
 
-int do_fncB(fame_plan* P, const double2* x, double2* y) {
+// One fncB over batch items [b0, b0 + nbc): z-pass + Pi mix, xy-pass, material, xy-pass, z-pass + Pi^H.
+int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y) {
   const long long N = P->N;
+  struct Window {  // restores the full-batch window on every exit path
+    fame_plan* P;
+    ~Window() { P->cb0 = 0; P->cnb = P->nb; }
+  } guard{P};
+  P->cb0 = b0;
+  P->cnb = nbc;
+  x += (size_t)b0 * 4 * N;
+  y += (size_t)b0 * 4 * N;
   const bool pr = P->prof;
   if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
   RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0));
@@ -370,8 +386,16 @@ int do_fncB(fame_plan* P, const double2* x, double2* y) {
       HIP_TRY(hipEventElapsedTime(&ms, P->ev[s], P->ev[s + 1]));
       P->stage_ms[s] += ms;
     }
-    P->prof_calls += 1;
   }
+  return FAME_OK;
+}
+
+// fncB on all nb items, in chunks so the intermediate fields of a chunk can stay
+// in the device's last-level cache between the five kernels.
+int do_fncB(fame_plan* P, const double2* x, double2* y) {
+  const int c = (P->chunk > 0 && P->chunk < P->nb) ? P->chunk : P->nb;
+  for (int b0 = 0; b0 < P->nb; b0 += c) RET_IF(fncB_window(P, b0, std::min(c, P->nb - b0), x, y));
+  if (P->prof) P->prof_calls += 1;
   return FAME_OK;
 }
 
@@ -480,6 +504,7 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   }
   fame_plan* P = new fame_plan();
   P->n1 = n1; P->n2 = n2; P->n3 = n3; P->nb = nb; P->N = N; P->n_mat = n_mat;
+  P->cb0 = 0; P->cnb = nb;
   P->tab_len = n1 + n2 + n3;
   P->theta.assign(theta, theta + 3 * nb);
   for (int d = 0; d < 3; ++d) P->delta[d] = delta[d];
@@ -600,6 +625,58 @@ int fame_plan_set_stream(fame_plan* P, void* stream) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   HIP_TRY(hipStreamSynchronize(P->stream));
   P->stream = (hipStream_t)stream;
+  return FAME_OK;
+}
+
+int fame_plan_set_chunk(fame_plan* P, int chunk) {
+  if (!P || chunk < 0) return set_err(FAME_ERR_ARG, "bad argument");
+  P->chunk = chunk;
+  return FAME_OK;
+}
+
+int fame_plan_get_chunk(fame_plan* P) { return P ? P->chunk : -1; }
+
+// Time fncB on the plan's own scratch for each candidate chunk and keep the fastest.
+int fame_plan_autotune(fame_plan* P, double* best_ms) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  const size_t bytes = (size_t)P->nb * 4 * P->N * sizeof(double2);
+  double2 *x = nullptr, *y = nullptr;
+  HIP_TRY(hipMalloc(&x, bytes));
+  HIP_TRY(hipMalloc(&y, bytes));
+  RET_IF(fill_zero(P, x, (long long)P->nb * 4 * P->N));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const bool prof = P->prof;
+  P->prof = false;
+  std::vector<int> cands;
+  for (int c : {1, 2, 3, 4, 6, 8, 12, 16})
+    if (c < P->nb) cands.push_back(c);
+  cands.push_back(0);
+  double best = 1e30;
+  int bestc = 0;
+  for (int c : cands) {
+    P->chunk = c;
+    RET_IF(do_fncB(P, x, y));  // warm-up
+    HIP_TRY(hipEventRecord(e0, P->stream));
+    const int reps = 5;
+    for (int r = 0; r < reps; ++r) RET_IF(do_fncB(P, x, y));
+    HIP_TRY(hipEventRecord(e1, P->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (ms / reps < best) {
+      best = ms / reps;
+      bestc = c;
+    }
+  }
+  P->chunk = bestc;
+  P->prof = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  if (best_ms) *best_ms = best;
   return FAME_OK;
 }
 

@@ -81,6 +81,9 @@ def load_library(path: str = LIB_PATH):
         "fame_dot": ([vp, vp, vp, ctypes.c_longlong, vp], i),
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
+        "fame_plan_set_chunk": ([vp, i], i),
+        "fame_plan_get_chunk": ([vp], i),
+        "fame_plan_autotune": ([vp, pd], i),
         "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
     }
     for name, (args, res) in sig.items():
@@ -98,7 +101,7 @@ def exported_symbols():
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_plan_profile", "fame_plan_stage_times",
-        "fame_plan_set_generic",
+        "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
     ]
 
 
@@ -183,6 +186,16 @@ class Plan:
 
     def sync(self):
         _check(self.lib.fame_sync(self.h))
+
+    def set_chunk(self, chunk):
+        """fncB processes the k-batch in chunks of ``chunk`` items (0 = all)."""
+        _check(self.lib.fame_plan_set_chunk(self.h, int(chunk)))
+
+    def autotune(self):
+        """Pick the fastest fncB chunk size on this plan; returns (chunk, ms per fncB)."""
+        ms = ctypes.c_double()
+        _check(self.lib.fame_plan_autotune(self.h, ctypes.byref(ms)))
+        return self.lib.fame_plan_get_chunk(self.h), ms.value
 
     def set_generic(self, generic=True):
         """Force the runtime-radix FFT kernels (True) or allow the specialised ones (False)."""

@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--nev", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chunk", type=int, default=-1,
+                    help="k-points per fncB chunk (-1: autotune on the plan, 0: whole batch)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -118,6 +120,12 @@ def main():
         if rc != 0:
             raise RuntimeError(lib.fame_last_error().decode())
 
+    if args.chunk < 0:
+        chunk, tune_ms = pb.autotune()
+        log(f"bench: autotuned chunk = {chunk} ({tune_ms:.3f} ms per batched fncB)")
+    else:
+        chunk = args.chunk
+        pb.set_chunk(chunk)
     x1, y1 = xb[0], yb[0]
 
     def run_plain():
@@ -191,7 +199,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"woodpile SC n={n} gamma=0.2, {nk}-k Gamma-X-M-R-Gamma band path, fncB batched over k",
-                   "n": n, "nk": nk, "vector_len": 4 * N},
+                   "n": n, "nk": nk, "vector_len": 4 * N, "chunk": chunk},
         "plain_matvecs_per_s": plain_rate,
         "batched_over_plain_matvec": value / plain_rate,
         "fncB_algo_GBs": fncb_gbs,

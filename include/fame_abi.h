@@ -77,6 +77,14 @@ void* fame_plan_stream(fame_plan* plan);
  * already queued on the previous stream is waited for first. */
 int fame_plan_set_stream(fame_plan* plan, void* stream);
 
+/* Batch chunking (no reference counterpart): fncB runs the nb k-points in chunks
+ * of `chunk` items (0 = all at once) so a chunk's intermediate fields stay in the
+ * last-level cache between kernels.  fame_plan_autotune times the candidate
+ * chunk sizes on the plan and keeps the fastest (best_ms may be NULL). */
+int fame_plan_set_chunk(fame_plan* plan, int chunk);
+int fame_plan_get_chunk(fame_plan* plan);
+int fame_plan_autotune(fame_plan* plan, double* best_ms);
+
 /* Kernel selection (no reference counterpart): grid lengths with a compile-time
  * specialised FFT (8,12,16,20,24,32,40,48,60,64,72,80,90,96; xy needs n1 == n2)
  * use it by default; generic != 0 forces the runtime-radix kernels that cover

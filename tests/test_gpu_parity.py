@@ -299,3 +299,22 @@ def test_specialised_fft_equals_generic(pkg, torch_mod, n):
     for a, b in zip(fast, gen):
         assert rel(a.cpu().numpy(), b.cpu().numpy()) < 1e-13
     plan.close()
+
+
+def test_chunked_fncB_equals_whole_batch(pkg, torch_mod):
+    say("chunked fncB == whole-batch fncB; autotune")
+    S = pkg.structures
+    dims = (16, 16, 16)
+    ks = S.sc_band_path()[::5]
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in ks])
+    table, ids = woodpile(pkg, dims)
+    x = torch_mod.as_tensor(crand(len(ks), 4 * np.prod(dims)))
+    plan = pkg.Plan(dims, thetas, table, ids)
+    ref = plan.fncB(x).cpu().numpy()
+    for c in (1, 3, 5):
+        plan.set_chunk(c)
+        assert np.array_equal(plan.fncB(x).cpu().numpy(), ref)
+    c, ms = plan.autotune()
+    assert c >= 0 and ms > 0
+    assert np.array_equal(plan.fncB(x).cpu().numpy(), ref)
+    plan.close()
