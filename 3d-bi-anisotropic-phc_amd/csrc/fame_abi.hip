@@ -14,6 +14,7 @@
 #include "../../include/fame_abi.h"
 #include "fame_kernels.hpp"
 #include "fame_fft_fast.hpp"
+#include "fame_material_z.hpp"
 
 using namespace fame;
 
@@ -96,6 +97,8 @@ struct fame_plan {
   // batch window used by the launch helpers: items [cb0, cb0 + cnb)
   int cb0 = 0, cnb = 0;
   int chunk = 0;              // fncB processes the batch in chunks of this many items (0: all)
+  int xyv = 0;                // xy-pass launch shape: 0 = default threads, 1 = 512 threads
+  int zv = 0;                 // z-pass launch shape: 0 = 16 pencils/256 thr, 1 = 8/128, 2 = 32/512
   // per-stage timing of fncB (HIP events between the five kernels), off by default
   bool prof = false;
   hipEvent_t ev[6] = {};
@@ -197,7 +200,6 @@ unsigned grid1d(long long len) {
   X(8, 2, 4) X(12, 3, 4) X(16, 4, 4) X(20, 4, 5) X(24, 4, 6) X(32, 4, 8) X(40, 5, 8) X(48, 6, 8) \
   X(60, 6, 10) X(64, 8, 8) X(72, 8, 9) X(80, 8, 10) X(90, 9, 10) X(96, 8, 12)
 
-constexpr int kZTP = 16;  // pencils per z work-group in the specialised pass
 
 template <int N>
 constexpr int xy_threads() { return N >= 72 ? 512 : 256; }
@@ -212,12 +214,10 @@ bool fast_size(int n) {
 bool use_fast_xy(const fame_plan* P) { return P->n1 == P->n2 && fast_size(P->n1); }
 bool use_fast_z(const fame_plan* P) { return fast_size(P->n3); }
 size_t xyfast_lds(int n) { return (size_t)(n * n + n) * sizeof(double2); }
-size_t zfast_lds(int n3) { return (size_t)(3 * n3 * kZTP + n3) * sizeof(double2); }
 
-template <int N, int A, int B>
+template <int N, int A, int B, int NT>
 int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs,
                    int nf, double sc) {
-  constexpr int NT = xy_threads<N>();
   const dim3 grid(P->n3, nf, P->cnb);
   const size_t lds = xyfast_lds(N);
   if (dir > 0) {
@@ -239,12 +239,11 @@ int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long lon
   return FAME_OK;
 }
 
-template <int N, int A, int B>
+template <int N, int A, int B, int kZTP, int NT>
 int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs,
                   int nslot, int grp_base, double sc) {
-  constexpr int NT = 256;
   const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->cnb);
-  const size_t lds = zfast_lds(N);
+  const size_t lds = (size_t)(3 * N * kZTP + N) * sizeof(double2);
   if (dir > 0) {
     if (mode == ZM_MIX_IN)
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
@@ -264,18 +263,33 @@ int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long 
   return FAME_OK;
 }
 
-template <int N, int A, int B>
-int fast_set_attrs() {
-  constexpr int NT = xy_threads<N>();
-  const int lx = (int)xyfast_lds(N), lz = (int)zfast_lds(N);
+template <int N, int A, int B, int NT>
+int xy_attrs() {
+  const int lx = (int)xyfast_lds(N);
   HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, 1, 0, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
   HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, 1, 1, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
   HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, -1, 0, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
   HIP_TRY(hipFuncSetAttribute((const void*)k_xyfast<N, A, B, -1, 2, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lx));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_PLAIN, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_PLAIN, kZTP, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  return FAME_OK;
+}
+
+template <int N, int A, int B, int TP, int NT>
+int z_attrs() {
+  const int lz = (int)((3 * N * TP + N) * sizeof(double2));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_MIX_IN, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_PLAIN, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_PLAIN, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  return FAME_OK;
+}
+
+template <int N, int A, int B>
+int fast_set_attrs() {
+  RET_IF((xy_attrs<N, A, B, xy_threads<N>()>()));
+  RET_IF((xy_attrs<N, A, B, 512>()));
+  RET_IF((z_attrs<N, A, B, 16, 256>()));
+  RET_IF((z_attrs<N, A, B, 8, 128>()));
+  RET_IF((z_attrs<N, A, B, 32, 512>()));
   return FAME_OK;
 }
 
@@ -291,8 +305,12 @@ int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, dou
           int grp_base) {
   const double sc = 1.0 / std::sqrt((double)P->n3);
   if (P->fast && use_fast_z(P)) {
-#define X(N, A, B) \
-    if (P->n3 == N) return z_fast_launch<N, A, B>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc);
+#define X(N, A, B)                                                                                       \
+    if (P->n3 == N) {                                                                                      \
+      if (P->zv == 1) return z_fast_launch<N, A, B, 8, 128>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc); \
+      if (P->zv == 2) return z_fast_launch<N, A, B, 32, 512>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc); \
+      return z_fast_launch<N, A, B, 16, 256>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc);        \
+    }
     FAME_FAST_SIZES(X)
 #undef X
   }
@@ -320,8 +338,11 @@ int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, dou
 int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs, int nf) {
   const double sc = 1.0 / std::sqrt((double)P->n1 * (double)P->n2);
   if (P->fast && use_fast_xy(P)) {
-#define X(N, A, B) \
-    if (P->n1 == N) return xy_fast_launch<N, A, B>(P, dir, twist, in, ibs, out, obs, nf, sc);
+#define X(N, A, B)                                                                                    \
+    if (P->n1 == N) {                                                                                   \
+      if (P->xyv == 1) return xy_fast_launch<N, A, B, 512>(P, dir, twist, in, ibs, out, obs, nf, sc);  \
+      return xy_fast_launch<N, A, B, xy_threads<N>()>(P, dir, twist, in, ibs, out, obs, nf, sc);       \
+    }
     FAME_FAST_SIZES(X)
 #undef X
   }
@@ -340,7 +361,39 @@ int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, d
   return FAME_OK;
 }
 
+// Tile of the 2.5-D material kernel: maximise (useful columns) x (tile / tile-with-halo) under
+// (TX+2)(TY+2) <= kMzThreads; z-slabs so that a launch has enough work-groups.
+void material_tile(int n1, int n2, int n3, int nbc, int& TX, int& TY, int& TZ) {
+  double best = -1;
+  TX = TY = 1;
+  for (int tx = 1; tx <= std::min(n1, kMzThreads); ++tx)
+    for (int ty = 1; ty <= std::min(n2, kMzThreads); ++ty) {
+      if ((tx + 2) * (ty + 2) > kMzThreads) break;
+      const double pad = (double)n1 * n2 / ((double)((n1 + tx - 1) / tx * tx) * ((n2 + ty - 1) / ty * ty));
+      const double eff = pad * tx * ty / (double)((tx + 2) * (ty + 2));
+      if (eff > best + 1e-12) {
+        best = eff;
+        TX = tx;
+        TY = ty;
+      }
+    }
+  const long long tiles = (long long)((n1 + TX - 1) / TX) * ((n2 + TY - 1) / TY) * nbc;
+  long long nzc = (2048 + tiles - 1) / tiles;
+  nzc = std::max<long long>(1, std::min<long long>(nzc, std::max(1, n3 / 4)));
+  TZ = (int)((n3 + nzc - 1) / nzc);
+}
+
 int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs) {
+  if (P->fast) {
+    int TX, TY, TZ;
+    material_tile(P->n1, P->n2, P->n3, P->cnb, TX, TY, TZ);
+    const int ntile = ((P->n1 + TX - 1) / TX) * ((P->n2 + TY - 1) / TY);
+    const int nzc = (P->n3 + TZ - 1) / TZ;
+    hipLaunchKernelGGL(k_material_z, dim3(ntile, nzc, P->cnb), dim3(kMzThreads), material_z_lds(TX, TY), P->stream,
+                       P->geo, in, ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, wom(P), TX, TY, TZ);
+    CHECK_LAUNCH();
+    return FAME_OK;
+  }
   const int tx = (P->n1 + kMatTX - 1) / kMatTX, ty = (P->n2 + kMatTY - 1) / kMatTY;
   const int tz = (P->n3 + kMatTZ - 1) / kMatTZ;
   hipLaunchKernelGGL(k_material, dim3(tx * ty, tz, P->cnb), dim3(kBlk), 0, P->stream, P->geo, in, ibs, out, obs,
@@ -577,6 +630,8 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<1, ZM_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_MIX_OUT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_material_z, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)material_z_lds_max()));
 #undef PLAN_HIP
   {
     int rc = FAME_OK;
@@ -636,6 +691,20 @@ int fame_plan_set_chunk(fame_plan* P, int chunk) {
 
 int fame_plan_get_chunk(fame_plan* P) { return P ? P->chunk : -1; }
 
+int fame_plan_set_variant(fame_plan* P, int xy_variant, int z_variant) {
+  if (!P || xy_variant < 0 || xy_variant > 1 || z_variant < 0 || z_variant > 2) return set_err(FAME_ERR_ARG, "bad variant");
+  P->xyv = xy_variant;
+  P->zv = z_variant;
+  return FAME_OK;
+}
+
+int fame_plan_get_variant(fame_plan* P, int* xy_variant, int* z_variant) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  if (xy_variant) *xy_variant = P->xyv;
+  if (z_variant) *z_variant = P->zv;
+  return FAME_OK;
+}
+
 // Time fncB on the plan's own scratch for each candidate chunk and keep the fastest.
 int fame_plan_autotune(fame_plan* P, double* best_ms) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
@@ -649,14 +718,7 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   HIP_TRY(hipEventCreate(&e1));
   const bool prof = P->prof;
   P->prof = false;
-  std::vector<int> cands;
-  for (int c : {1, 2, 3, 4, 6, 8, 12, 16})
-    if (c < P->nb) cands.push_back(c);
-  cands.push_back(0);
-  double best = 1e30;
-  int bestc = 0;
-  for (int c : cands) {
-    P->chunk = c;
+  auto time_it = [&](double& ms_out) -> int {
     RET_IF(do_fncB(P, x, y));  // warm-up
     HIP_TRY(hipEventRecord(e0, P->stream));
     const int reps = 5;
@@ -665,10 +727,31 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    if (ms / reps < best) {
-      best = ms / reps;
-      bestc = c;
+    ms_out = ms / reps;
+    return FAME_OK;
+  };
+  double best = 1e30;
+  int bestc = 0, bestxy = 0, bestz = 0;
+  // launch shapes first (whole batch), then the chunk size with the best shapes
+  P->chunk = 0;
+  for (int xv = 0; xv < 2; ++xv)
+    for (int zv = 0; zv < 3; ++zv) {
+      P->xyv = xv;
+      P->zv = zv;
+      double ms;
+      RET_IF(time_it(ms));
+      if (ms < best) { best = ms; bestxy = xv; bestz = zv; }
     }
+  P->xyv = bestxy;
+  P->zv = bestz;
+  std::vector<int> cands;
+  for (int c : {1, 2, 3, 4, 6, 8, 12, 16})
+    if (c < P->nb) cands.push_back(c);
+  for (int c : cands) {
+    P->chunk = c;
+    double ms;
+    RET_IF(time_it(ms));
+    if (ms < best) { best = ms; bestc = c; }
   }
   P->chunk = bestc;
   P->prof = prof;

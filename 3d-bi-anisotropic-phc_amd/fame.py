@@ -84,6 +84,8 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_set_chunk": ([vp, i], i),
         "fame_plan_get_chunk": ([vp], i),
         "fame_plan_autotune": ([vp, pd], i),
+        "fame_plan_set_variant": ([vp, i, i], i),
+        "fame_plan_get_variant": ([vp, pi_, pi_], i),
         "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
     }
     for name, (args, res) in sig.items():
@@ -102,6 +104,7 @@ def exported_symbols():
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_plan_profile", "fame_plan_stage_times",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
+        "fame_plan_set_variant", "fame_plan_get_variant",
     ]
 
 
@@ -192,10 +195,19 @@ class Plan:
         _check(self.lib.fame_plan_set_chunk(self.h, int(chunk)))
 
     def autotune(self):
-        """Pick the fastest fncB chunk size on this plan; returns (chunk, ms per fncB)."""
+        """Pick the fastest FFT launch shapes and fncB chunk size on this plan;
+        returns (chunk, ms per fncB)."""
         ms = ctypes.c_double()
         _check(self.lib.fame_plan_autotune(self.h, ctypes.byref(ms)))
         return self.lib.fame_plan_get_chunk(self.h), ms.value
+
+    def set_variant(self, xy_variant, z_variant):
+        _check(self.lib.fame_plan_set_variant(self.h, int(xy_variant), int(z_variant)))
+
+    def variant(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.fame_plan_get_variant(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def set_generic(self, generic=True):
         """Force the runtime-radix FFT kernels (True) or allow the specialised ones (False)."""

@@ -122,7 +122,8 @@ def main():
 
     if args.chunk < 0:
         chunk, tune_ms = pb.autotune()
-        log(f"bench: autotuned chunk = {chunk} ({tune_ms:.3f} ms per batched fncB)")
+        log(f"bench: autotuned chunk = {chunk}, xy/z launch variants = {pb.variant()} "
+            f"({tune_ms:.3f} ms per batched fncB)")
     else:
         chunk = args.chunk
         pb.set_chunk(chunk)
@@ -199,7 +200,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"woodpile SC n={n} gamma=0.2, {nk}-k Gamma-X-M-R-Gamma band path, fncB batched over k",
-                   "n": n, "nk": nk, "vector_len": 4 * N, "chunk": chunk},
+                   "n": n, "nk": nk, "vector_len": 4 * N, "chunk": chunk,
+                   "fft_variants": list(pb.variant())},
         "plain_matvecs_per_s": plain_rate,
         "batched_over_plain_matvec": value / plain_rate,
         "fncB_algo_GBs": fncb_gbs,

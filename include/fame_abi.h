@@ -85,10 +85,18 @@ int fame_plan_set_chunk(fame_plan* plan, int chunk);
 int fame_plan_get_chunk(fame_plan* plan);
 int fame_plan_autotune(fame_plan* plan, double* best_ms);
 
+/* Launch shapes of the specialised FFT passes (no reference counterpart):
+ * xy_variant 0 = default threads per plane work-group, 1 = 512 threads;
+ * z_variant 0 = 16 pencils / 256 threads, 1 = 8 / 128, 2 = 32 / 512.
+ * fame_plan_autotune picks these first, then the chunk size. */
+int fame_plan_set_variant(fame_plan* plan, int xy_variant, int z_variant);
+int fame_plan_get_variant(fame_plan* plan, int* xy_variant, int* z_variant);
+
 /* Kernel selection (no reference counterpart): grid lengths with a compile-time
  * specialised FFT (8,12,16,20,24,32,40,48,60,64,72,80,90,96; xy needs n1 == n2)
- * use it by default; generic != 0 forces the runtime-radix kernels that cover
- * every supported size (used to cross-check the two). */
+ * use it by default, and the material stencil runs as a 2.5-D z-march;
+ * generic != 0 forces the runtime-radix FFT kernels and the tiled 3-D stencil
+ * that cover every supported size (used to cross-check the two). */
 int fame_plan_set_generic(fame_plan* plan, int generic);
 
 /* Measurement aid (no reference counterpart): with enable != 0 every fame_fncB

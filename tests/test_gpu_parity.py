@@ -318,3 +318,21 @@ def test_chunked_fncB_equals_whole_batch(pkg, torch_mod):
     assert c >= 0 and ms > 0
     assert np.array_equal(plan.fncB(x).cpu().numpy(), ref)
     plan.close()
+
+
+@pytest.mark.parametrize("n", [24, 60])
+def test_launch_variants_identical(pkg, torch_mod, n):
+    """Every FFT launch-shape variant computes the same numbers (bitwise)."""
+    say(f"launch variants n={n}")
+    S = pkg.structures
+    dims = (n, n, n)
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in S.sc_band_path()[:3]])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    x = torch_mod.as_tensor(crand(3, 4 * plan.N))
+    ref = plan.fncB(x).cpu().numpy()
+    for xv in (0, 1):
+        for zv in (0, 1, 2):
+            plan.set_variant(xv, zv)
+            assert np.array_equal(plan.fncB(x).cpu().numpy(), ref), (xv, zv)
+    plan.close()
