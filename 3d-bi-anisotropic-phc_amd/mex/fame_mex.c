@@ -1,0 +1,185 @@
+/* MATLAB MEX gateway over libfame (include/fame_abi.h).
+ *
+ * Source only: neither MATLAB nor mex.h exists in this build's environment, so
+ * this file is not compiled or tested here.  Build inside MATLAB with
+ *   mex -R2018a -I<repo>/include fame_mex.c -L<pkg> -lfame -lamdhip64
+ * (interleaved complex API, R2018a+).
+ *
+ * MATLAB holds the vectors on the host (gpuArray is CUDA-only), so every
+ * operator call copies its input to a device buffer owned by the plan slot and
+ * its output back: PCIe traffic per call.  The solver entry points (fncinvB,
+ * lanczos) keep all iterations on the device and copy only their operands.
+ *
+ *   h = fame_mex('plan_create', [n1 n2 n3], theta, delta, mat_table, mat_id)
+ *        theta     nb x 3 double       (k . a_d per k-point)
+ *        delta     1 x 3 double        (opt.comp.discrete.mesh_len)
+ *        mat_table n_mat x 27 complex  (N-, M-, J-slot 3x3 tensors, row-major)
+ *        mat_id    N x 1 uint8         (0-based material per vertex)
+ *   y = fame_mex('fncA' | 'fncB', h, x)           x: 4N x nb complex
+ *   y = fame_mex('Psi' | 'Psis', h, x)            x: 3N x nb complex (Psi == Phi for QP)
+ *   y = fame_mex('Pr' | 'Qr', h, x)               x: 2N x nb complex -> 3N x nb
+ *   y = fame_mex('Prs' | 'Qrs', h, x)             x: 3N x nb complex -> 2N x nb
+ *   [y, iters, relres] = fame_mex('fncinvB', h, x, tol, maxit)
+ *   [ev, ew, iters] = fame_mex('lanczos', h, k, tol, maxiter, p, v0)   v0: 4N x nb
+ *   fame_mex('plan_destroy', h)
+ */
+#include <stdint.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "fame_abi.h"
+#include "mex.h"
+
+typedef struct {
+  fame_plan* plan;
+  long long N;
+  int nb;
+  void* din;   /* device staging, 6N x nb complex */
+  void* dout;  /* device staging, 6N x nb complex */
+} slot_t;
+
+static slot_t* slot_of(const mxArray* a) {
+  if (!mxIsUint64(a) || mxGetNumberOfElements(a) != 1) mexErrMsgIdAndTxt("fame:handle", "bad plan handle");
+  return (slot_t*)(uintptr_t)(*(uint64_t*)mxGetData(a));
+}
+
+static void check(int rc) {
+  if (rc != FAME_OK && rc != FAME_ERR_NOT_CONVERGED) mexErrMsgIdAndTxt("fame:call", "%s", fame_last_error());
+}
+
+static void hip_check(hipError_t e) {
+  if (e != hipSuccess) mexErrMsgIdAndTxt("fame:hip", "%s", hipGetErrorString(e));
+}
+
+/* host complex (interleaved) -> device staging buffer */
+static void to_dev(slot_t* s, const mxArray* x, long long len, void* dst) {
+  if (!mxIsComplex(x)) mexErrMsgIdAndTxt("fame:arg", "complex input expected");
+  if ((long long)mxGetNumberOfElements(x) != len * s->nb) mexErrMsgIdAndTxt("fame:arg", "wrong vector length");
+  hip_check(hipMemcpy(dst, mxGetComplexDoubles(x), (size_t)len * s->nb * 16, hipMemcpyHostToDevice));
+}
+
+static mxArray* from_dev(slot_t* s, const void* src, long long len) {
+  mxArray* y = mxCreateDoubleMatrix((mwSize)len, (mwSize)s->nb, mxCOMPLEX);
+  check(fame_sync(s->plan));
+  hip_check(hipMemcpy(mxGetComplexDoubles(y), src, (size_t)len * s->nb * 16, hipMemcpyDeviceToHost));
+  return y;
+}
+
+typedef int (*op_t)(fame_plan*, const void*, void*);
+
+static void apply(slot_t* s, op_t op, const mxArray* x, long long lin, long long lout, mxArray** y) {
+  to_dev(s, x, lin, s->din);
+  check(op(s->plan, s->din, s->dout));
+  *y = from_dev(s, s->dout, lout);
+}
+
+static int psi(fame_plan* p, const void* a, void* b) { return fame_MtxVecMul_FFT(p, 0, a, b); }
+static int psis(fame_plan* p, const void* a, void* b) { return fame_MtxVecMul_FFT(p, 1, a, b); }
+
+void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
+  char cmd[32];
+  if (nrhs < 1 || mxGetString(prhs[0], cmd, sizeof cmd)) mexErrMsgIdAndTxt("fame:arg", "first argument: command");
+  if (!strcmp(cmd, "plan_create")) {
+    if (nrhs != 6) mexErrMsgIdAndTxt("fame:arg", "plan_create(dims, theta, delta, mat_table, mat_id)");
+    const double* d = mxGetDoubles(prhs[1]);
+    const int n1 = (int)d[0], n2 = (int)d[1], n3 = (int)d[2];
+    const int nb = (int)mxGetM(prhs[2]);
+    /* theta arrives column-major nb x 3; the ABI wants [nb][3] */
+    double* th = (double*)mxMalloc(sizeof(double) * 3 * nb);
+    const double* tm = mxGetDoubles(prhs[2]);
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < 3; ++c) th[3 * b + c] = tm[b + nb * c];
+    const int n_mat = (int)mxGetM(prhs[4]);
+    /* mat_table column-major n_mat x 27 -> [n_mat][27] interleaved */
+    double* tab = (double*)mxMalloc(sizeof(double) * 54 * n_mat);
+    const mxComplexDouble* tc = mxGetComplexDoubles(prhs[4]);
+    for (int m = 0; m < n_mat; ++m)
+      for (int w = 0; w < 27; ++w) {
+        tab[2 * (27 * m + w)] = tc[m + n_mat * w].real;
+        tab[2 * (27 * m + w) + 1] = tc[m + n_mat * w].imag;
+      }
+    slot_t* s = (slot_t*)mxCalloc(1, sizeof(slot_t));
+    mexMakeMemoryPersistent(s);
+    check(fame_plan_create(&s->plan, n1, n2, n3, nb, th, mxGetDoubles(prhs[3]), n_mat, tab,
+                           (const unsigned char*)mxGetData(prhs[5])));
+    s->N = (long long)n1 * n2 * n3;
+    s->nb = nb;
+    hip_check(hipMalloc(&s->din, (size_t)6 * s->N * nb * 16));
+    hip_check(hipMalloc(&s->dout, (size_t)6 * s->N * nb * 16));
+    mxFree(th);
+    mxFree(tab);
+    plhs[0] = mxCreateNumericMatrix(1, 1, mxUINT64_CLASS, mxREAL);
+    *(uint64_t*)mxGetData(plhs[0]) = (uint64_t)(uintptr_t)s;
+    return;
+  }
+  if (nrhs < 2) mexErrMsgIdAndTxt("fame:arg", "missing plan handle");
+  slot_t* s = slot_of(prhs[1]);
+  const long long N = s->N;
+  if (!strcmp(cmd, "plan_destroy")) {
+    fame_plan_destroy(s->plan);
+    hipFree(s->din);
+    hipFree(s->dout);
+    mxFree(s);
+  } else if (!strcmp(cmd, "fncA")) {
+    apply(s, fame_fncA, prhs[2], 4 * N, 4 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "fncB")) {
+    apply(s, fame_fncB, prhs[2], 4 * N, 4 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "Psi") || !strcmp(cmd, "Phi")) {
+    apply(s, psi, prhs[2], 3 * N, 3 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "Psis") || !strcmp(cmd, "Phis")) {
+    apply(s, psis, prhs[2], 3 * N, 3 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "Pr")) {
+    apply(s, fame_Pr, prhs[2], 2 * N, 3 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "Qr")) {
+    apply(s, fame_Qr, prhs[2], 2 * N, 3 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "Prs")) {
+    apply(s, fame_Prs, prhs[2], 3 * N, 2 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "Qrs")) {
+    apply(s, fame_Qrs, prhs[2], 3 * N, 2 * N, &plhs[0]);
+  } else if (!strcmp(cmd, "fncinvB")) {
+    const double tol = mxGetScalar(prhs[3]);
+    const int maxit = (int)mxGetScalar(prhs[4]);
+    int* it = (int*)mxMalloc(sizeof(int) * s->nb);
+    double* rr = (double*)mxMalloc(sizeof(double) * s->nb);
+    to_dev(s, prhs[2], 4 * N, s->din);
+    check(fame_fncinvB(s->plan, s->din, s->dout, tol, maxit, it, rr));
+    plhs[0] = from_dev(s, s->dout, 4 * N);
+    if (nlhs > 1) {
+      plhs[1] = mxCreateDoubleMatrix(1, s->nb, mxREAL);
+      for (int b = 0; b < s->nb; ++b) mxGetDoubles(plhs[1])[b] = it[b];
+    }
+    if (nlhs > 2) {
+      plhs[2] = mxCreateDoubleMatrix(1, s->nb, mxREAL);
+      memcpy(mxGetDoubles(plhs[2]), rr, sizeof(double) * s->nb);
+    }
+    mxFree(it);
+    mxFree(rr);
+  } else if (!strcmp(cmd, "lanczos")) {
+    const int k = (int)mxGetScalar(prhs[2]);
+    const double tol = mxGetScalar(prhs[3]);
+    const int maxiter = (int)mxGetScalar(prhs[4]);
+    const int p = (int)mxGetScalar(prhs[5]);
+    void* dev_ev = NULL;
+    hip_check(hipMalloc(&dev_ev, (size_t)k * 4 * N * s->nb * 16));
+    to_dev(s, prhs[6], 4 * N, s->din);
+    double* ew = (double*)mxMalloc(sizeof(double) * k * s->nb);
+    int* it = (int*)mxMalloc(sizeof(int) * s->nb);
+    check(fame_lanczos(s->plan, k, tol, maxiter, p, s->din, 1e-12, 10000, dev_ev, ew, it));
+    plhs[0] = mxCreateDoubleMatrix((mwSize)(4 * N), (mwSize)(k * s->nb), mxCOMPLEX);
+    hip_check(hipMemcpy(mxGetComplexDoubles(plhs[0]), dev_ev, (size_t)k * 4 * N * s->nb * 16, hipMemcpyDeviceToHost));
+    hipFree(dev_ev);
+    if (nlhs > 1) {
+      plhs[1] = mxCreateDoubleMatrix((mwSize)k, (mwSize)s->nb, mxREAL);
+      memcpy(mxGetDoubles(plhs[1]), ew, sizeof(double) * k * s->nb);
+    }
+    if (nlhs > 2) {
+      plhs[2] = mxCreateDoubleMatrix(1, s->nb, mxREAL);
+      for (int b = 0; b < s->nb; ++b) mxGetDoubles(plhs[2])[b] = it[b];
+    }
+    mxFree(ew);
+    mxFree(it);
+  } else {
+    mexErrMsgIdAndTxt("fame:arg", "unknown command %s", cmd);
+  }
+}
