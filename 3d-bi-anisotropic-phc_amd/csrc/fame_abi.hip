@@ -213,7 +213,7 @@ bool fast_size(int n) {
 
 bool use_fast_xy(const fame_plan* P) { return P->n1 == P->n2 && fast_size(P->n1); }
 bool use_fast_z(const fame_plan* P) { return fast_size(P->n3); }
-size_t xyfast_lds(int n) { return (size_t)(n * n + n) * sizeof(double2); }
+size_t xyfast_lds(int n) { return (size_t)(n * (n % 2 == 0 ? n + 1 : n) + n) * sizeof(double2); }
 
 template <int N, int A, int B, int NT>
 int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs,

@@ -74,9 +74,11 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
                                                const double2* __restrict__ tws, int tws_bstride,
                                                const double2* __restrict__ twg, double scale) {
   static_assert(A * B == N, "k_xyfast: N = A*B");
+  // odd row pitch: lanes that walk down a column hit distinct LDS banks
+  constexpr int PT = (N % 2 == 0) ? N + 1 : N;
   extern __shared__ double2 sm[];
   double2* pl = sm;
-  double2* tw = sm + N * N;
+  double2* tw = sm + N * PT;
   const int i3 = blockIdx.x, f = blockIdx.y, b = blockIdx.z;
   const long long base = (long long)f * g.N + (long long)i3 * (N * N);
   const double2* __restrict__ src = in + b * in_bstride + base;
@@ -101,22 +103,22 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
       u[j] = v;
     }
     rdft<A, DIR>(u, tw, B);
-    pl[r * N + t] = u[0];
+    pl[r * PT + t] = u[0];
 #pragma unroll
-    for (int k = 1; k < A; ++k) pl[r * N + t + B * k] = twmul<DIR>(u[k], tw[t * k]);
+    for (int k = 1; k < A; ++k) pl[r * PT + t + B * k] = twmul<DIR>(u[k], tw[t * k]);
   }
   __syncthreads();
 
-  // x phase B: item (r, k); read B values, hold, barrier, write natural order k + A q
+  // x phase B: item (r, k), row fastest; read B values, hold, barrier, write natural order k + A q
   constexpr int IB = (N * A + NT - 1) / NT;
   double2 ub[IB][B];
 #pragma unroll
   for (int i = 0; i < IB; ++i) {
     const int it = threadIdx.x + i * NT;
     if (it < N * A) {
-      const int r = it / A, k = it - (it / A) * A;
+      const int k = it / N, r = it - (it / N) * N;
 #pragma unroll
-      for (int t = 0; t < B; ++t) ub[i][t] = pl[r * N + t + B * k];
+      for (int t = 0; t < B; ++t) ub[i][t] = pl[r * PT + t + B * k];
     }
   }
   __syncthreads();
@@ -124,10 +126,10 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
   for (int i = 0; i < IB; ++i) {
     const int it = threadIdx.x + i * NT;
     if (it < N * A) {
-      const int r = it / A, k = it - (it / A) * A;
+      const int k = it / N, r = it - (it / N) * N;
       rdft<B, DIR>(ub[i], tw, A);
 #pragma unroll
-      for (int q = 0; q < B; ++q) pl[r * N + k + A * q] = ub[i][q];
+      for (int q = 0; q < B; ++q) pl[r * PT + k + A * q] = ub[i][q];
     }
   }
   __syncthreads();
@@ -137,11 +139,11 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
     const int t = it / N, c = it - (it / N) * N;
     double2 u[A];
 #pragma unroll
-    for (int j = 0; j < A; ++j) u[j] = pl[(t + B * j) * N + c];
+    for (int j = 0; j < A; ++j) u[j] = pl[(t + B * j) * PT + c];
     rdft<A, DIR>(u, tw, B);
-    pl[t * N + c] = u[0];
+    pl[t * PT + c] = u[0];
 #pragma unroll
-    for (int k = 1; k < A; ++k) pl[(t + B * k) * N + c] = twmul<DIR>(u[k], tw[t * k]);
+    for (int k = 1; k < A; ++k) pl[(t + B * k) * PT + c] = twmul<DIR>(u[k], tw[t * k]);
   }
   __syncthreads();
 
@@ -150,7 +152,7 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
     const int k = it / N, c = it - (it / N) * N;
     double2 u[B];
 #pragma unroll
-    for (int t = 0; t < B; ++t) u[t] = pl[(B * k + t) * N + c];
+    for (int t = 0; t < B; ++t) u[t] = pl[(B * k + t) * PT + c];
     rdft<B, DIR>(u, tw, A);
     double2 sxz = make_double2(1, 0);
     if (TWIST == 1) sxz = cmul(S[c], sz);
