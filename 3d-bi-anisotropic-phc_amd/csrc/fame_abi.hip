@@ -94,8 +94,19 @@ struct fame_plan {
   CgState* d_cg = nullptr;
   int tab_len = 0;            // n1+n2+n3
   bool fast = true;           // use the specialised FFT kernels where a size has them
-  // batch window used by the launch helpers: items [cb0, cb0 + cnb)
+  // batch window used by the launch helpers: items [cb0, cb0 + cnb) of the current view
   int cb0 = 0, cnb = 0;
+  // view: the batch the operators see.  By default all nb items with the plan's tables;
+  // a packed view (set_view) holds m < nb items whose table rows were gathered into
+  // d_*_v, so solvers can run only on the k-points that are still iterating.
+  int vnb = 0;
+  const double2* v_lam = nullptr;
+  const double2* v_tws = nullptr;
+  const double2* v_om = nullptr;
+  double2* d_lam_v = nullptr;
+  double2* d_tws_v = nullptr;
+  double2* d_om_v = nullptr;
+  int* d_idx = nullptr;       // [nb] item indices of the packed view
   int chunk = 0;              // fncB processes the batch in chunks of this many items (0: all)
   int xyv = 0;                // xy-pass launch shape: 0 = default threads, 1 = 512 threads
   int zv = 0;                 // z-pass launch shape: 0 = 16 pencils/256 thr, 1 = 8/128, 2 = 32/512
@@ -108,9 +119,9 @@ struct fame_plan {
 
 namespace {
 
-inline const double2* wlam(const fame_plan* P) { return P->d_lam + (size_t)P->cb0 * P->tab_len; }
-inline const double2* wtws(const fame_plan* P) { return P->d_tws + (size_t)P->cb0 * P->tab_len; }
-inline const double2* wom(const fame_plan* P) { return P->d_omega + (size_t)P->cb0 * 3; }
+inline const double2* wlam(const fame_plan* P) { return P->v_lam + (size_t)P->cb0 * P->tab_len; }
+inline const double2* wtws(const fame_plan* P) { return P->v_tws + (size_t)P->cb0 * P->tab_len; }
+inline const double2* wom(const fame_plan* P) { return P->v_om + (size_t)P->cb0 * 3; }
 
 int build_axis(int n, std::vector<double2>& tw, std::vector<int>& perm, AxisPlan& ap) {
   std::vector<int> r;
@@ -414,7 +425,7 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y) {
   const long long N = P->N;
   struct Window {  // restores the full-batch window on every exit path
     fame_plan* P;
-    ~Window() { P->cb0 = 0; P->cnb = P->nb; }
+    ~Window() { P->cb0 = 0; P->cnb = P->vnb; }
   } guard{P};
   P->cb0 = b0;
   P->cnb = nbc;
@@ -446,14 +457,64 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y) {
 // fncB on all nb items, in chunks so the intermediate fields of a chunk can stay
 // in the device's last-level cache between the five kernels.
 int do_fncB(fame_plan* P, const double2* x, double2* y) {
-  const int c = (P->chunk > 0 && P->chunk < P->nb) ? P->chunk : P->nb;
-  for (int b0 = 0; b0 < P->nb; b0 += c) RET_IF(fncB_window(P, b0, std::min(c, P->nb - b0), x, y));
+  const int nb = P->vnb;
+  const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
+  for (int b0 = 0; b0 < nb; b0 += c) RET_IF(fncB_window(P, b0, std::min(c, nb - b0), x, y));
   if (P->prof) P->prof_calls += 1;
   return FAME_OK;
 }
 
+// rows of `len` complex words: dst[j] = src[idx[j]] for j < m (blockIdx.y = j)
+__global__ void k_gather_rows(const double2* __restrict__ src, double2* __restrict__ dst, const int* __restrict__ idx,
+                              long long len, int scatter) {
+  const int j = blockIdx.y;
+  const long long si = scatter ? j : idx[j], di = scatter ? idx[j] : j;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)
+    dst[di * len + i] = src[si * len + i];
+}
+
+void reset_view(fame_plan* P) {
+  P->vnb = P->nb;
+  P->cnb = P->nb;
+  P->cb0 = 0;
+  P->v_lam = P->d_lam;
+  P->v_tws = P->d_tws;
+  P->v_om = P->d_omega;
+}
+
+// Operators see only `items` (ascending item indices) from now on, packed 0..m-1.
+int set_view(fame_plan* P, const std::vector<int>& items) {
+  const int m = (int)items.size();
+  if (m < 1 || m > P->nb) return set_err(FAME_ERR_ARG, "set_view: bad item count");
+  if (m == P->nb) {
+    reset_view(P);
+    return FAME_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(P->d_idx, items.data(), m * sizeof(int), hipMemcpyHostToDevice, P->stream));
+  hipLaunchKernelGGL(k_gather_rows, dim3(1, m), dim3(64), 0, P->stream, P->d_lam, P->d_lam_v, P->d_idx,
+                     (long long)P->tab_len, 0);
+  hipLaunchKernelGGL(k_gather_rows, dim3(1, m), dim3(64), 0, P->stream, P->d_tws, P->d_tws_v, P->d_idx,
+                     (long long)P->tab_len, 0);
+  hipLaunchKernelGGL(k_gather_rows, dim3(1, m), dim3(64), 0, P->stream, P->d_omega, P->d_om_v, P->d_idx, 3LL, 0);
+  CHECK_LAUNCH();
+  HIP_TRY(hipStreamSynchronize(P->stream));  // items' host buffer is reused by the caller
+  P->vnb = m;
+  P->cnb = m;
+  P->v_lam = P->d_lam_v;
+  P->v_tws = P->d_tws_v;
+  P->v_om = P->d_om_v;
+  return FAME_OK;
+}
+
+// pack (scatter = 0: dst[j] = src[items[j]]) or unpack (scatter = 1) vectors of `len` words
+int pack_rows(fame_plan* P, const double2* src, double2* dst, long long len, int m, int scatter) {
+  hipLaunchKernelGGL(k_gather_rows, dim3(grid1d(len), m), dim3(kBlk), 0, P->stream, src, dst, P->d_idx, len, scatter);
+  CHECK_LAUNCH();
+  return FAME_OK;
+}
+
 int do_fncA(fame_plan* P, const double2* x, double2* y) {
-  hipLaunchKernelGGL(k_fncA, dim3(grid1d(P->N), P->nb), dim3(kBlk), 0, P->stream, P->geo, x, y, P->d_lam,
+  hipLaunchKernelGGL(k_fncA, dim3(grid1d(P->N), P->vnb), dim3(kBlk), 0, P->stream, P->geo, x, y, P->v_lam,
                      P->tab_len);
   CHECK_LAUNCH();
   return FAME_OK;
@@ -484,8 +545,8 @@ int copy(fame_plan* P, const double2* x, double2* y, long long len) {
 
 int do_cg(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int* iters, double* relres) {
   const long long L = 4 * P->N;
-  const long long tot = L * P->nb;
-  const int nb = P->nb;
+  const long long tot = L * P->vnb;
+  const int nb = P->vnb;
   RET_IF(fill_zero(P, x, tot));
   RET_IF(copy(P, b, P->d_r, tot));
   RET_IF(copy(P, b, P->d_p, tot));
@@ -557,7 +618,7 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   }
   fame_plan* P = new fame_plan();
   P->n1 = n1; P->n2 = n2; P->n3 = n3; P->nb = nb; P->N = N; P->n_mat = n_mat;
-  P->cb0 = 0; P->cnb = nb;
+  P->cb0 = 0; P->cnb = nb; P->vnb = nb;
   P->tab_len = n1 + n2 + n3;
   P->theta.assign(theta, theta + 3 * nb);
   for (int d = 0; d < 3; ++d) P->delta[d] = delta[d];
@@ -590,6 +651,13 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   PLAN_HIP(hipMalloc(&P->d_lam, (size_t)nb * P->tab_len * sizeof(double2)));
   PLAN_HIP(hipMalloc(&P->d_tws, (size_t)nb * P->tab_len * sizeof(double2)));
   PLAN_HIP(hipMalloc(&P->d_omega, (size_t)nb * 3 * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_lam_v, (size_t)nb * P->tab_len * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_tws_v, (size_t)nb * P->tab_len * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_om_v, (size_t)nb * 3 * sizeof(double2)));
+  PLAN_HIP(hipMalloc(&P->d_idx, (size_t)nb * sizeof(int)));
+  P->v_lam = P->d_lam;
+  P->v_tws = P->d_tws;
+  P->v_om = P->d_omega;
   // material: symmetrise eps and mu as 1/2 (X + X^H) (the symmetrisation of N_int, M_int at
   // Interpolation_Material_Matrix_Mult_Point_vertex.m:22,24 acts on the vertex tensors like this)
   std::vector<double2> mt((size_t)n_mat * kMatWords);
@@ -655,7 +723,8 @@ int fame_plan_destroy(fame_plan* P) {
   if (!P) return FAME_OK;
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
-                  P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg};
+                  P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
+                  P->d_idx};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)

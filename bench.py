@@ -77,7 +77,8 @@ def main():
     ap.add_argument("--nk", type=int, default=40, help="k-points of the band path (batch size)")
     ap.add_argument("--solve", choices=["none", "plain", "both"], default="both",
                     help="also time full Lanczos solves (s per k-point)")
-    ap.add_argument("--solve-k", type=int, default=1, help="k-points solved one by one for the plain s/k-point")
+    ap.add_argument("--solve-k", type=int, default=8,
+                    help="k-points (spread over the path) solved one by one for the plain s/k-point")
     ap.add_argument("--nev", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -210,37 +211,42 @@ def main():
         "roofline": roofline,
     }
 
-    # ---- s per k-point: full Lanczos solves ----
+    # ---- s per k-point: full Lanczos solves (table setup + eigensolve, as the reference's time_eig window) ----
     if args.solve != "none":
         k = args.nev
-        times = []
-        iters = []
-        for j in range(args.solve_k):
+        sample = np.unique(np.linspace(0, nk - 1, min(args.solve_k, nk)).round().astype(int))
+        times, steps = [], []
+        for j in sample:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            pj = pkg.Plan(dims, thetas[j % nk], table, ids)
+            pj = pkg.Plan(dims, thetas[j], table, ids)
             ev, ew, it = pj.Lanczos(k, tol=1e-10, maxiter=10000, p=2 * k, cg_tol=1e-12, cg_maxit=10000)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-            iters.append(int(it[0]))
-            omega = np.sort(1.0 / ew[0])
+            steps.append(int(it[0]))
             pj.close()
             del ev
-            log(f"bench: plain solve k#{j}: {times[-1]:.2f} s, {iters[-1]} Lanczos steps, omega[:3]={omega[:3]}")
+            log(f"bench: plain solve k#{j}: {times[-1]:.2f} s, {steps[-1]} Lanczos steps, "
+                f"omega[:3]={np.sort(1.0 / ew[0])[:3]}")
         out["s_per_kpoint_plain"] = float(np.mean(times))
-        out["lanczos_steps_plain"] = iters
+        out["plain_solve_sample"] = {"k_indices": sample.tolist(), "seconds": times, "lanczos_steps": steps,
+                                     "note": "mean over this sample of the band path, one k-point per plan"}
         if args.solve == "both":
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             pl = pkg.Plan(dims, thetas, table, ids)
+            pl.set_chunk(chunk)
             ev, ew, it = pl.Lanczos(k, tol=1e-10, maxiter=10000, p=2 * k, cg_tol=1e-12, cg_maxit=10000)
             torch.cuda.synchronize()
             tb = time.perf_counter() - t0
             pl.close()
             del ev
             out["s_per_kpoint_batched"] = tb / nk
+            out["batched_solve_seconds"] = tb
+            out["lanczos_steps_batched"] = it.tolist()
             out["batched_over_plain_solve"] = out["s_per_kpoint_plain"] / out["s_per_kpoint_batched"]
-            log(f"bench: batched solve {nk} k: {tb:.2f} s -> {tb / nk:.3f} s/k")
+            log(f"bench: batched solve {nk} k: {tb:.2f} s -> {tb / nk:.3f} s/k "
+                f"(x{out['batched_over_plain_solve']:.2f} vs plain)")
 
     if not args.no_cpu:
         log("bench: CPU baseline")

@@ -336,3 +336,24 @@ def test_launch_variants_identical(pkg, torch_mod, n):
             plan.set_variant(xv, zv)
             assert np.array_equal(plan.fncB(x).cpu().numpy(), ref), (xv, zv)
     plan.close()
+
+
+def test_batched_lanczos_equals_plain(pkg, torch_mod):
+    """Lock-step batched Lanczos (k-points finishing at different steps, solved on a
+    packed view of the unfinished ones) gives each k-point's plain result."""
+    say("batched Lanczos == plain Lanczos per k")
+    S = pkg.structures
+    dims = (8, 8, 8)
+    ks = S.sc_band_path()[[0, 5, 17, 26]]
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in ks])
+    table, ids = woodpile(pkg, dims)
+    pb = pkg.Plan(dims, thetas, table, ids)
+    evb, ewb, itb = pb.Lanczos(10, tol=1e-10, maxiter=10000)
+    assert len(set(itb.tolist())) > 1  # items really finish at different steps
+    for b in range(len(ks)):
+        p1 = pkg.Plan(dims, thetas[b], table, ids)
+        ev1, ew1, it1 = p1.Lanczos(10, tol=1e-10, maxiter=10000)
+        assert it1[0] == itb[b]
+        assert np.allclose(ew1[0], ewb[b], rtol=1e-12, atol=0)
+        p1.close()
+    pb.close()
