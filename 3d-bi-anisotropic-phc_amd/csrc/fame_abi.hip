@@ -937,3 +937,4 @@ int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit,
 }  // extern "C"
 
 #include "fame_lanczos.inc"
+#include "fame_minres.inc"

@@ -79,6 +79,9 @@ def load_library(path: str = LIB_PATH):
         "fame_lanczos": ([vp, i, d, i, i, vp, d, i, vp, pd, pi_], i),
         "fame_eigen_restoration": ([vp, i, vp, vp], i),
         "fame_dot": ([vp, vp, vp, ctypes.c_longlong, vp], i),
+        "fame_minres": ([vp, vp, vp, d, d, i, pi_, pd], i),
+        "fame_block_dot": ([vp, vp, i, vp, ctypes.c_longlong, vp], i),
+        "fame_block_gemv": ([vp, vp, i, ctypes.c_longlong, vp, d, vp], i),
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
         "fame_plan_set_chunk": ([vp, i], i),
@@ -102,7 +105,8 @@ def exported_symbols():
         "fame_version", "fame_last_error", "fame_plan_create", "fame_plan_set_theta", "fame_plan_destroy",
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
-        "fame_eigen_restoration", "fame_dot", "fame_plan_profile", "fame_plan_stage_times",
+        "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv",
+        "fame_plan_profile", "fame_plan_stage_times",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
     ]
@@ -287,6 +291,38 @@ class Plan:
             _check(rc)
         y = y.reshape(-1) if self.nb == 1 else y
         return y, np.array(iters[:]), np.array(relres[:])
+
+    def minres(self, x, sigma=0.0, tol=1e-2, maxit=30000, raise_on_maxit=False):
+        """MINRES on (fncB - sigma fncA) y = x (the SIRA inner solve); returns (y, iters, relres)."""
+        xin = self._as_dev(x, 4)
+        y = self.empty(4)
+        iters = (ctypes.c_int * self.nb)()
+        relres = (ctypes.c_double * self.nb)()
+        rc = self.lib.fame_minres(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                  float(sigma), float(tol), int(maxit), iters, relres)
+        if rc != FAME_ERR_NOT_CONVERGED or raise_on_maxit:
+            _check(rc)
+        y = y.reshape(-1) if self.nb == 1 else y
+        return y, np.array(iters[:]), np.array(relres[:])
+
+    def block_dot(self, V, w):
+        """V^H w for V (m, len) and w (len,), device tensors; returns a device tensor (m,)."""
+        m, L = V.shape
+        out = self.torch.empty(m, dtype=self.torch.complex128, device=self.device)
+        _check(self.lib.fame_block_dot(self.h, ctypes.c_void_p(V.data_ptr()), int(m), ctypes.c_void_p(w.data_ptr()),
+                                       int(L), ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def block_gemv(self, V, y, out=None, beta=0.0):
+        """out = beta*out + V^T-combination sum_j y_j V_j for V (m, len); y device (m,)."""
+        m, L = V.shape
+        if out is None:
+            out = self.torch.empty(L, dtype=self.torch.complex128, device=self.device)
+            beta = 0.0
+        y = y.to(device=self.device, dtype=self.torch.complex128).contiguous()
+        _check(self.lib.fame_block_gemv(self.h, ctypes.c_void_p(V.data_ptr()), int(m), int(L),
+                                        ctypes.c_void_p(y.data_ptr()), float(beta), ctypes.c_void_p(out.data_ptr())))
+        return out
 
     def Lanczos(self, k, tol=1e-10, maxiter=10000, p=None, v0=None, cg_tol=1e-12, cg_maxit=10000):
         """Thick-restart Lanczos (Lanczos.m:1-156); returns (ev [nb,k,4N], ew [nb,k], iters [nb])."""

@@ -152,6 +152,23 @@ int fame_fncinvB(fame_plan* plan, const void* in, void* out, double tol, int max
 int fame_lanczos(fame_plan* plan, int k, double tol, int maxiter, int p, const void* v0, double cg_tol, int cg_maxit,
                  void* ev, double* ew, int* iters);
 
+/* MINRES on (fncB - sigma fncA) y = x for all nb items in lock step, from y0 = 0;
+ * stops when the MINRES residual estimate |eta| <= tol ||x|| for every item.
+ * Replaces the inner solve `minres1(@(vec)GEP_shift_mtx_prod_vec(vec, A, B, sigma), rhs, tol, maxit)`
+ * of light_locked_Huang/SIRA_solve_residual_LS_GEP.m:54 (with A = fncB, B = fncA as
+ * the JD-SIRA drivers pass them); sigma = 0 is the reference's operator
+ * (Jacobi_Davidson/GEP_shift_mtx_prod_vec.m:28 applies A only).
+ * Returns FAME_ERR_NOT_CONVERGED (result still written) if maxit is reached. */
+int fame_minres(fame_plan* plan, const void* in, void* out, double sigma, double tol, int maxit, int* iters,
+                double* relres);
+
+/* Tall-skinny projections for Rayleigh-Ritz and Gram-Schmidt on one vector set
+ * (light_locked_Huang/GEP_AB_Herm_JDSIRA_Locking4.m:93,97,172,326-340):
+ *   fame_block_dot : out[j] = V_j^H w, j < m  (V: m columns of len, column-major; out on the device)
+ *   fame_block_gemv: out = beta * out + V y   (y: m complex on the device) */
+int fame_block_dot(fame_plan* plan, const void* V, int m, const void* w, long long len, void* out);
+int fame_block_gemv(fame_plan* plan, const void* V, int m, long long len, const void* y, double beta, void* out);
+
 /* Eigen_Restoration_Chiral_no_inv (fun_no_inv/Eigen_Restoration_Chiral_no_inv.m:3-25):
  * for nvec NFSEP vectors per item ([nb][nvec][4N]) returns [nb][nvec][6N] =
  * [e (3N); h (3N)] with e = -i (N Qr(v_b) + J Pr(v_a)), h = -i (J^H Qr(v_b) + M Pr(v_a)). */

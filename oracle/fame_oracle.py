@@ -31,6 +31,9 @@ Pinning status (what checks each function; see ``tests/test_oracle_pins.py``):
   non-zero spectrum of the original curl problem built only from the explicit
   curl (EvaluteError_Chiral_no_inv.m:43-44) by dense eig at tiny n.
 * ``fncinvB``: ||B y - x|| <= tol ||x||.
+* ``minres``: the k-th iterate equals the least-squares minimiser of ||b - A x||
+  over the explicit Krylov space K_k(A, b) (brute force at tiny size), and the
+  converged solution equals a dense solve.
 * ``lanczos``: vacuum closed form omega = sqrt(Lambda_q) and dense ``eigh`` of the
   full GEP at tiny n.
 * ``eigen_restoration`` / ``evaluate_error``: original-problem residual small for
@@ -468,6 +471,49 @@ def pcg(op, b, tol, maxit, x0=None):
         p = r + (rho_new / rho) * p
         rho = rho_new
     return x, 1, np.linalg.norm(b - op(x)) / nb, maxit
+
+
+def minres(op, b, tol, maxit):
+    """Unpreconditioned MINRES (Paige & Saunders 1975) for a Hermitian operator.
+
+    The SIRA inner solver (light_locked_Huang/SIRA_solve_residual_LS_GEP.m:50-53)
+    calls ``minres1``, which the reference does not ship; this is the published
+    algorithm: Lanczos tridiagonalisation, Givens QR of the tridiagonal, and the
+    three-term update of the search directions w.  Stops when the recursive
+    residual |eta_k| <= tol*||b||.  Returns (x, flag, relres, iters)."""
+    n = b.shape[0]
+    x = np.zeros(n, complex)
+    beta1 = np.linalg.norm(b)
+    if beta1 == 0:
+        return x, 0, 0.0, 0
+    v_old = np.zeros(n, complex)
+    v = b / beta1
+    w1 = np.zeros(n, complex)   # w_{k-1}
+    w2 = np.zeros(n, complex)   # w_{k-2}
+    beta = 0.0
+    c1, s1, c2, s2 = 1.0, 0.0, 1.0, 0.0  # rotations k-1 and k-2
+    eta = beta1
+    for k in range(1, maxit + 1):
+        p = op(v)
+        alpha = np.vdot(v, p).real
+        p = p - alpha * v - beta * v_old
+        beta_new = np.linalg.norm(p)
+        delta = c1 * alpha - c2 * s1 * beta      # rotated diagonal of column k
+        rho2 = s1 * alpha + c2 * c1 * beta       # entry k-1
+        rho3 = s2 * beta                         # entry k-2
+        rho1 = np.hypot(delta, beta_new)
+        c, s = (delta / rho1, beta_new / rho1) if rho1 > 0 else (1.0, 0.0)
+        w = (v - rho3 * w2 - rho2 * w1) / rho1 if rho1 > 0 else np.zeros(n, complex)
+        x = x + c * eta * w
+        eta = -s * eta
+        w2, w1 = w1, w
+        c2, s2, c1, s1 = c1, s1, c, s
+        v_old = v
+        v = p / beta_new if beta_new > 1e-300 else np.zeros(n, complex)
+        beta = beta_new
+        if abs(eta) <= tol * beta1 or beta_new <= 1e-300 or rho1 == 0:
+            return x, 0, abs(eta) / beta1, k
+    return x, 1, abs(eta) / beta1, maxit
 
 
 def fncinvB(x, B_op, tol, maxit):

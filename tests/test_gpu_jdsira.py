@@ -1,0 +1,127 @@
+"""GPU parity of the JD-SIRA pieces: block projections, the device MINRES (the SIRA
+inner solve) and the JD-SIRA driver, through the C-ABI, against the oracle.
+
+Tolerances: projections 1e-12 relative; MINRES iterates against the oracle's
+MINRES after the same number of steps 1e-10 relative (same algorithm, rounding
+grows with the step count); eigenvalues 1e-10 relative to dense eigh; residual
+||A u - w B u|| <= 10 * stop_tolerance (the driver's acceptance rule)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(77)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def woodpile_plan(pkg, fo, dims, thetas, gamma=0.2):
+    S = pkg.structures
+    table, ids = S.bi_isotropic_table(13.0, gamma, S.SC_LATTICE), S.woodpile_sc_ids(dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    T = table[ids]
+    probs = [fo.Problem(dims, th, T[:, 0], T[:, 1], T[:, 2]) for th in np.atleast_2d(thetas)]
+    return plan, probs
+
+
+THETAS = np.array([[0.4, -1.3, 2.2], [1.5, 0.1, -0.7]])
+
+
+@pytest.mark.parametrize("m,length", [(1, 480), (7, 480), (12, 4 * 1000 + 4), (3, 4 * 13824)])
+def test_block_dot_gemv(pkg, torch_mod, m, length):
+    say(f"block_dot/gemv m={m} len={length}")
+    t = torch_mod
+    plan = pkg.Plan((4, 5, 6), THETAS[0], np.zeros((1, 3, 3, 3), complex) + np.eye(3), np.zeros(120, np.uint8))
+    V = crand(m, length)
+    w = crand(length)
+    y = crand(m)
+    base = crand(length)
+    Vd, wd = t.as_tensor(V, device="cuda"), t.as_tensor(w, device="cuda")
+    d = plan.block_dot(Vd, wd).cpu().numpy()
+    assert rel(d, V.conj() @ w) < 1e-12
+    g = plan.block_gemv(Vd, t.as_tensor(y)).cpu().numpy()
+    assert rel(g, y @ V) < 1e-12
+    out = t.as_tensor(base, device="cuda")
+    plan.block_gemv(Vd, t.as_tensor(y), out=out, beta=-0.5)
+    assert rel(out.cpu().numpy(), -0.5 * base + y @ V) < 1e-12
+    plan.close()
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.8])
+def test_minres_iterates_match_oracle(pkg, fo, torch_mod, sigma):
+    say(f"minres iterates sigma={sigma}")
+    dims = (6, 5, 4)
+    plan, probs = woodpile_plan(pkg, fo, dims, THETAS)
+    x = crand(2, 4 * plan.N)
+    for steps in (1, 4, 15):
+        y, iters, relres = plan.minres(torch_mod.as_tensor(x), sigma=sigma, tol=1e-30, maxit=steps)
+        y = y.cpu().numpy()
+        for b in range(2):
+            P = probs[b]
+            yo, flag, ro, ito = fo.minres(lambda v: P.fncB(v) - sigma * P.fncA(v), x[b], 1e-30, steps)
+            assert iters[b] == steps == ito
+            assert rel(y[b], yo) < 1e-10, (steps, b)
+            assert abs(relres[b] - ro) <= 1e-8 * ro
+    plan.close()
+
+
+def test_minres_converges(pkg, fo, torch_mod):
+    say("minres converged solve, batched")
+    dims = (8, 8, 8)
+    plan, probs = woodpile_plan(pkg, fo, dims, THETAS)
+    x = crand(2, 4 * plan.N)
+    y, iters, relres = plan.minres(torch_mod.as_tensor(x), tol=1e-11, maxit=3000)
+    y = y.cpu().numpy()
+    for b in range(2):
+        P = probs[b]
+        assert relres[b] <= 1e-11 and 0 < iters[b] < 3000
+        assert np.linalg.norm(P.fncB(y[b]) - x[b]) <= 1e-10 * np.linalg.norm(x[b])
+    with pytest.raises(pkg.NotConverged):
+        plan.minres(torch_mod.as_tensor(x), tol=1e-11, maxit=3, raise_on_maxit=True)
+    y, iters, _ = plan.minres(torch_mod.as_tensor(x), tol=1e-11, maxit=3)
+    assert list(iters) == [3, 3]
+    plan.close()
+
+
+def test_jdsira_woodpile_matches_dense(pkg, fo, torch_mod):
+    say("JD-SIRA woodpile n=6 vs dense GEP")
+    dims = (6, 6, 6)
+    S = pkg.structures
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    plan, (P,) = woodpile_plan(pkg, fo, dims, theta)
+    n4 = 4 * P.N
+    Ad = np.stack([P.fncA(e) for e in np.eye(n4, dtype=complex)], 1)
+    Bd = np.stack([P.fncB(e) for e in np.eye(n4, dtype=complex)], 1)
+    mu = scipy.linalg.eigh(Ad, Bd, eigvals_only=True)
+    k = 4
+    want = np.sort(1.0 / mu[mu > 0])[:k]
+    res = pkg.jdsira.GEP_AB_Herm_JDSIRA_Driver4(plan, k, stop_tolerance=1e-8, seed=0)
+    got = np.sort(res.ew.real)
+    say(f"  omega {got} inner {res.total_inner} outer {res.outer_iterations}")
+    assert np.allclose(got, want, rtol=1e-10)
+    EV = res.ev.cpu().numpy()
+    for j in range(k):
+        u = EV[j]
+        assert abs(np.linalg.norm(u) - 1) < 1e-12
+        assert np.linalg.norm(Bd @ u - res.ew[j] * (Ad @ u)) <= 1e-7
+    plan.close()

@@ -241,3 +241,31 @@ def test_pcg_residual(fo):
     y, flag, relres, it = fo.pcg(P.fncB, x, 1e-12, 1000)
     assert flag == 0
     assert np.linalg.norm(P.fncB(y) - x) <= 1e-12 * np.linalg.norm(x) * 1.01
+
+
+@pytest.mark.parametrize("shifted", [False, True])
+def test_minres_iterates_are_krylov_least_squares(fo, shifted):
+    """x_k = argmin ||b - A x|| over K_k(A, b): brute force with an explicit basis.
+    Shifted: A Hermitian indefinite (the A - sigma B inner operator)."""
+    rng = np.random.default_rng(11)
+    n = 40
+    Q = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    d = rng.uniform(0.5, 3.0, n) * (np.where(np.arange(n) % 3 == 0, -1.0, 1.0) if shifted else 1.0)
+    A = Q @ np.diag(d) @ Q.conj().T
+    A = 0.5 * (A + A.conj().T)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for k in (1, 2, 5, 9):
+        x, flag, relres, it = fo.minres(lambda v: A @ v, b, 1e-30, k)
+        assert it == k and flag == 1
+        K = np.empty((n, k), complex)
+        K[:, 0] = b
+        for j in range(1, k):
+            K[:, j] = A @ K[:, j - 1]
+        Kq = np.linalg.qr(K)[0]
+        y = np.linalg.lstsq(A @ Kq, b, rcond=None)[0]
+        xls = Kq @ y
+        assert np.linalg.norm(x - xls) <= 1e-9 * np.linalg.norm(xls)
+        # the recursive residual estimate is the true residual
+        assert abs(relres - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) < 1e-9
+    x, flag, relres, it = fo.minres(lambda v: A @ v, b, 1e-12, 500)
+    assert flag == 0 and np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-10 * np.linalg.norm(x)
