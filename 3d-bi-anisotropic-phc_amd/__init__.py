@@ -5,5 +5,5 @@ with ``importlib`` under an alias such as ``phc_amd`` — see ``load_package`` i
 ``__graft_entry__.py``).
 """
 from . import jdsira, structures  # noqa: F401
-from .fame import (FameError, NotConverged, Plan, exported_symbols, load_library,  # noqa: F401
+from .fame import (FameError, FccPlan, NotConverged, Plan, exported_symbols, load_library,  # noqa: F401
                    LIB_PATH)

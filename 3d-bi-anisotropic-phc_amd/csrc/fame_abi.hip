@@ -938,3 +938,4 @@ int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit,
 
 #include "fame_lanczos.inc"
 #include "fame_minres.inc"
+#include "fame_fcc.inc"

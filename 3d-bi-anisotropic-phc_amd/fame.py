@@ -90,6 +90,22 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_set_variant": ([vp, i, i], i),
         "fame_plan_get_variant": ([vp, pi_, pi_], i),
         "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
+        "fame_fcc_create": ([ctypes.POINTER(vp), i, i, i, pd, pd, pd, pd, pd], i),
+        "fame_fcc_destroy": ([vp], i),
+        "fame_fcc_set_stream": ([vp, vp], i),
+        "fame_fcc_sync": ([vp], i),
+        "fame_fcc_T": ([vp, i, vp, vp], i),
+        "fame_fcc_TH": ([vp, i, vp, vp], i),
+        "fame_fcc_Sigma_r": ([vp, vp], i),
+        "fame_fcc_Pr": ([vp, vp, vp], i),
+        "fame_fcc_Qr": ([vp, vp, vp], i),
+        "fame_fcc_Prs": ([vp, vp, vp], i),
+        "fame_fcc_Qrs": ([vp, vp, vp], i),
+        "fame_fcc_nfgep_A": ([vp, vp, vp, vp, vp, vp], i),
+        "fame_fcc_eigenvector": ([vp, vp, vp, vp, vp, vp], i),
+        "fame_fcc_set_material": ([vp, vp, vp, pd], i),
+        "fame_fcc_param_apply": ([vp, i, vp, vp], i),
+        "fame_fcc_gep_B": ([vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -109,6 +125,9 @@ def exported_symbols():
         "fame_plan_profile", "fame_plan_stage_times",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
+        "fame_fcc_create", "fame_fcc_destroy", "fame_fcc_set_stream", "fame_fcc_sync", "fame_fcc_T", "fame_fcc_TH",
+        "fame_fcc_Sigma_r", "fame_fcc_Pr", "fame_fcc_Qr", "fame_fcc_Prs", "fame_fcc_Qrs", "fame_fcc_nfgep_A",
+        "fame_fcc_eigenvector", "fame_fcc_set_material", "fame_fcc_param_apply", "fame_fcc_gep_B",
     ]
 
 
@@ -363,3 +382,157 @@ class Plan:
                   + np.abs(np.kron(lam[2], np.ones(n2 * n1))) ** 2)
             out.append(np.concatenate([np.sqrt(lq), np.sqrt(lq)]))
         return np.stack(out)
+
+
+class FccPlan:
+    """FCC skewed-grid operators for one grid and one k (light_locked_Huang/ FCC path).
+
+    ``SVDSingleCurl_FCC(n1, n2, n3, vec_k, a1, a2, a3, delta)`` builds the tables
+    (light_locked_Huang/SVDSingleCurl_FCC.m:1); the methods mirror
+    ``p_mtx_T_prod_vec_FCC`` / ``p_mtx_TH_prod_vec_FCC`` (batched over rows),
+    ``mtx_prod_vec_A_NFGEP`` (diagonal Phi), ``Compute_eigenvector``,
+    ``mtx_Eps_Mu_Zeta_Xi`` + ``Construct_GEP_mtx_B`` (as operators).  Vectors are
+    complex128 torch tensors on the GPU.
+    """
+
+    PARAMS = ("Eps_e", "Eps_f", "Mu_e", "Mu_f", "Zeta_e", "Zeta_f", "Xi_e", "Xi_f")
+
+    def __init__(self, dims, vec_k, lattice, delta, device=None):
+        import torch
+
+        self.lib = load_library()
+        self.torch = torch
+        self.device = torch.device("cuda" if device is None else device)
+        self.dims = tuple(int(d) for d in dims)
+        self.n = int(np.prod(self.dims))
+        arr = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))  # noqa: E731
+        k, a1, a2, a3, dl = arr(vec_k), *(arr(a) for a in lattice), arr(delta)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(self.lib.fame_fcc_create(ctypes.byref(h), *self.dims, _dptr(k), _dptr(a1), _dptr(a2), _dptr(a3),
+                                            _dptr(dl)))
+        self.h = h
+        _check(self.lib.fame_fcc_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fame_fcc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _dev(self, x, length=None):
+        t = self.torch
+        if not isinstance(x, t.Tensor):
+            x = t.as_tensor(np.asarray(x))
+        x = x.to(device=self.device, dtype=t.complex128).contiguous()
+        if length is not None and x.numel() != length:
+            raise ValueError(f"expected {length} complex values, got {x.numel()}")
+        return x
+
+    def _empty(self, length):
+        return self.torch.empty(length, dtype=self.torch.complex128, device=self.device)
+
+    def _vec(self, fn, x, lin, lout):
+        xin = self._dev(x, lin)
+        y = self._empty(lout)
+        _check(fn(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr())))
+        return y
+
+    def T(self, x):
+        """p_mtx_T_prod_vec_FCC on each row of x (nvec, n) or on x (n,)."""
+        return self._batched(self.lib.fame_fcc_T, x)
+
+    def TH(self, x):
+        """p_mtx_TH_prod_vec_FCC on each row of x (nvec, n) or on x (n,)."""
+        return self._batched(self.lib.fame_fcc_TH, x)
+
+    def _batched(self, fn, x):
+        xin = self._dev(x)
+        if xin.numel() % self.n:
+            raise ValueError("length must be a multiple of n1*n2*n3")
+        nvec = xin.numel() // self.n
+        y = self.torch.empty_like(xin)
+        _check(fn(self.h, nvec, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr())))
+        return y
+
+    def Sigma_r(self):
+        s = self.torch.empty(8 * self.n, dtype=self.torch.float64, device=self.device)
+        _check(self.lib.fame_fcc_Sigma_r(self.h, ctypes.c_void_p(s.data_ptr())))
+        return s
+
+    def Pr(self, u):
+        return self._vec(self.lib.fame_fcc_Pr, u, 16 * self.n, 24 * self.n)
+
+    def Qr(self, v):
+        return self._vec(self.lib.fame_fcc_Qr, v, 16 * self.n, 24 * self.n)
+
+    def Prs(self, y):
+        return self._vec(self.lib.fame_fcc_Prs, y, 24 * self.n, 16 * self.n)
+
+    def Qrs(self, y):
+        return self._vec(self.lib.fame_fcc_Qrs, y, 24 * self.n, 16 * self.n)
+
+    def _coef(self, c):
+        t = self.torch
+        if np.isscalar(c) or (isinstance(c, t.Tensor) and c.numel() == 1) or np.ndim(c) == 0:
+            return t.full((24 * self.n,), complex(c), dtype=t.complex128, device=self.device)
+        return self._dev(c, 24 * self.n)
+
+    def mtx_prod_vec_A_NFGEP(self, x, xi_d, zeta_d, Phi):
+        """mtx_prod_vec_A_NFGEP (mtx_prod_vec_A_NFGEP.m:1) with diagonal Phi; xi_d, zeta_d, Phi are
+        scalars or 24n vectors."""
+        xin = self._dev(x, 32 * self.n)
+        c = [self._coef(v) for v in (xi_d, zeta_d, Phi)]
+        y = self._empty(32 * self.n)
+        _check(self.lib.fame_fcc_nfgep_A(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                         *(ctypes.c_void_p(v.data_ptr()) for v in c)))
+        return y
+
+    def Compute_eigenvector(self, x, xi_d, zeta_d, Phi):
+        """Compute_eigenvector (Compute_eigenvector.m:1) with diagonal Phi: 32n -> 48n."""
+        xin = self._dev(x, 32 * self.n)
+        c = [self._coef(v) for v in (xi_d, zeta_d, Phi)]
+        y = self._empty(48 * self.n)
+        _check(self.lib.fame_fcc_eigenvector(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                             *(ctypes.c_void_p(v.data_ptr()) for v in c)))
+        return y
+
+    @staticmethod
+    def masks(n, sets):
+        """Bit masks from four index sets (X, Y, Z, vertex-or-Cubic): 0-based index arrays or
+        boolean masks of length n."""
+        m = np.zeros(n, np.uint8)
+        for bit, s in enumerate(sets):
+            s = np.asarray(s)
+            sel = s.astype(bool) if (s.dtype == bool and s.size == n) else s.astype(np.int64)
+            m[sel] |= np.uint8(1 << bit)
+        return m
+
+    def mtx_Eps_Mu_Zeta_Xi(self, edge_sets, face_sets, tensors_in, tensors_air):
+        """Set the material of mtx_Eps_Mu_Zeta_Xi.m:1 (edge sets X, Y, Z, Cubic; face sets X, Y, Z,
+        vertex; tensors (eps, mu, zeta, xi) 3x3 inside and in air)."""
+        em = np.ascontiguousarray(self.masks(self.n, edge_sets))
+        fm = np.ascontiguousarray(self.masks(self.n, face_sets))
+        ten = np.stack([np.stack([np.asarray(T, complex).reshape(3, 3) for T in tensors_in]),
+                        np.stack([np.asarray(T, complex).reshape(3, 3) for T in tensors_air])])
+        tv = np.ascontiguousarray(ten.view(np.float64).reshape(-1))
+        _check(self.lib.fame_fcc_set_material(self.h, em.ctypes.data_as(ctypes.c_void_p),
+                                              fm.ctypes.data_as(ctypes.c_void_p), _dptr(tv)))
+
+    def param_apply(self, name, x):
+        """One of the eight matrices of mtx_Eps_Mu_Zeta_Xi.m:23-35 applied to x (12n)."""
+        which = self.PARAMS.index(name)
+        xin = self._dev(x, 12 * self.n)
+        y = self._empty(12 * self.n)
+        _check(self.lib.fame_fcc_param_apply(self.h, which, ctypes.c_void_p(xin.data_ptr()),
+                                             ctypes.c_void_p(y.data_ptr())))
+        return y
+
+    def GEP_B(self, x):
+        """GEP_mtx_B x (Construct_GEP_mtx_B.m:1), 48n."""
+        return self._vec(self.lib.fame_fcc_gep_B, x, 48 * self.n, 48 * self.n)

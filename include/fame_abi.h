@@ -178,6 +178,66 @@ int fame_eigen_restoration(fame_plan* plan, int nvec, const void* ev, void* e_h)
  * written to device memory (nb complex). */
 int fame_dot(fame_plan* plan, const void* x, const void* y, long long len, void* out);
 
+/* ---------------------------------------------------------------------------
+ * FCC skewed-grid operators (the reference's light_locked_Huang/ FCC path).
+ *
+ * One plan per (grid, k).  Grid vectors of length n = n1*n2*n3.  The input of T
+ * (and output of T^H) is in frequency order, z fastest: p = l + n3*(nn + n2*m)
+ * (p_mtx_T_prod_vec_FCC.m:9 reshapes it as n3 x n1*n2); the output of T is in grid
+ * order, x fastest.  Sizes: Lambda_Pr/Qr map 16n -> 24n; the NFGEP vector is 32n;
+ * the parameter matrices act on 12n; GEP B on 48n.  All vector pointers are
+ * device pointers; calls are asynchronous on the plan's stream.
+ * ------------------------------------------------------------------------- */
+typedef struct fame_fcc fame_fcc;
+
+/* Tables of SVDSingleCurl_FCC (light_locked_Huang/SVDSingleCurl_FCC.m:1-137):
+ * vec_k[3], lattice vectors a1, a2, a3 (period_vec1..3), delta[3].  The explicit
+ * difference matrices this factorises need n1 even and n2 divisible by 3
+ * (Construct_GEP_mtx_A.m:451-505); T itself does not. */
+int fame_fcc_create(fame_fcc** plan, int n1, int n2, int n3, const double* vec_k, const double* a1, const double* a2,
+                    const double* a3, const double* delta);
+int fame_fcc_destroy(fame_fcc* plan);
+/* stream: a hipStream_t, NULL = the default stream */
+int fame_fcc_set_stream(fame_fcc* plan, void* stream);
+int fame_fcc_sync(fame_fcc* plan);
+
+/* v = p_mtx_T_prod_vec_FCC(v, FFT_parameter) on nvec vectors [nvec][n]
+ * (light_locked_Huang/p_mtx_T_prod_vec_FCC.m:1-37); in == out allowed. */
+int fame_fcc_T(fame_fcc* plan, int nvec, const void* in, void* out);
+/* v = p_mtx_TH_prod_vec_FCC(v, FFT_parameter_bar) (light_locked_Huang/p_mtx_TH_prod_vec_FCC.m:1-28). */
+int fame_fcc_TH(fame_fcc* plan, int nvec, const void* in, void* out);
+/* SVD_curl.Sigma_r (SVDSingleCurl_FCC.m:66-69): 8n doubles, device. */
+int fame_fcc_Sigma_r(fame_fcc* plan, void* out);
+/* (I_24 (x) T) SVD_curl.Lambda_Pr u and (I_24 (x) T) SVD_curl.Lambda_Qr v, 16n -> 24n
+ * (mtx_prod_vec_A_NFGEP.m:12-24); Prs/Qrs are the adjoints 24n -> 16n (:36-50). */
+int fame_fcc_Pr(fame_fcc* plan, const void* in, void* out);
+int fame_fcc_Qr(fame_fcc* plan, const void* in, void* out);
+int fame_fcc_Prs(fame_fcc* plan, const void* in, void* out);
+int fame_fcc_Qrs(fame_fcc* plan, const void* in, void* out);
+/* output_vec = mtx_prod_vec_A_NFGEP(n, input_vec, SVD_curl, mtx_data, TH, T)
+ * (light_locked_Huang/mtx_prod_vec_A_NFGEP.m:1-52), 32n -> 32n, with
+ * mtx_data.Phi_is_diag = 'yes'.  xi_d, zeta_d, Phi: 24n complex device arrays (the
+ * diagonals of mtx_data.xi_d, zeta_d and the vector mtx_data.Phi). */
+int fame_fcc_nfgep_A(fame_fcc* plan, const void* in, void* out, const void* xi_d, const void* zeta_d,
+                     const void* Phi);
+/* ev = Compute_eigenvector(GEP_mtx_B, n, input_vec, SVD_curl, mtx_data, T), 32n -> 48n
+ * (light_locked_Huang/Compute_eigenvector.m:1-43), diagonal Phi. */
+int fame_fcc_eigenvector(fame_fcc* plan, const void* in, void* out, const void* xi_d, const void* zeta_d,
+                         const void* Phi);
+/* Material of mtx_Eps_Mu_Zeta_Xi (light_locked_Huang/mtx_Eps_Mu_Zeta_Xi.m:1-38): host arrays.
+ * edge_mask[n]: bit 0/1/2 = point in edge_idx_in.X/Y/Z, bit 3 = in Cubic_idx_in;
+ * face_mask[n]: bit 0/1/2 = face_idx_in.X/Y/Z, bit 3 = vertex_idx_in.
+ * tensors: [2][4][3][3] complex, [inside, air][eps, mu, zeta, xi], row-major 3x3
+ * (mtx_par_* and val_*_air). */
+int fame_fcc_set_material(fame_fcc* plan, const unsigned char* edge_mask, const unsigned char* face_mask,
+                          const double* tensors);
+/* out = M in for one of the eight matrices of mtx_Eps_Mu_Zeta_Xi.m:23-35
+ * (Construct_parameter_mtx.m:1-318), 12n -> 12n; which = 2*q + f with q = 0 eps,
+ * 1 mu, 2 zeta, 3 xi and f = 0 edge, 1 face. */
+int fame_fcc_param_apply(fame_fcc* plan, int which, const void* in, void* out);
+/* out = GEP_mtx_B in, 48n (light_locked_Huang/Construct_GEP_mtx_B.m:1-69). */
+int fame_fcc_gep_B(fame_fcc* plan, const void* in, void* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,140 @@
+"""GPU parity of the FCC skewed-grid operators (C-ABI fame_fcc_*) against the FCC oracle
+(oracle/fcc_oracle.py, itself pinned by unitarity, the closed form and Gamma_d T = T Lambda_d).
+
+Tolerance: 1e-12 relative (FP64 FFT-based operators)."""
+import numpy as np
+import pytest
+
+from oracle import fcc_oracle as fc
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+rng = np.random.default_rng(31)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def make(pkg, dims, k):
+    n1, n2, n3 = dims
+    lat = fc.fcc_lattice()
+    delta = np.array([1 / n1, np.sqrt(3) / 2 / n2, np.sqrt(6) / 3 / n3])
+    tab = fc.fft_tables(n1, n2, n3, k, *lat, delta)
+    return pkg.FccPlan(dims, k, lat, delta), tab
+
+
+K = np.array([0.31, -0.17, 0.23])
+DIMS = [(2, 3, 3), (4, 6, 6), (6, 3, 4), (12, 12, 12), (10, 9, 8)]
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_T_TH_match_oracle(pkg, torch_mod, dims):
+    say(f"FCC T/TH dims={dims}")
+    plan, tab = make(pkg, dims, K)
+    n = plan.n
+    x = crand(3, n)
+    y = plan.T(torch_mod.as_tensor(x)).cpu().numpy()
+    z = plan.TH(torch_mod.as_tensor(x)).cpu().numpy()
+    for r in range(3):
+        assert rel(y[r], fc.T(x[r], tab)) < RTOL
+        assert rel(z[r], fc.TH(x[r], tab)) < RTOL
+    plan.close()
+
+
+def test_T_batches_beyond_one_launch_and_k0(pkg, torch_mod):
+    say("FCC T nvec=50, k=0")
+    dims = (4, 6, 6)
+    plan, tab = make(pkg, dims, np.zeros(3))
+    x = crand(50, plan.n)
+    y = plan.T(torch_mod.as_tensor(x)).cpu().numpy()
+    assert max(rel(y[r], fc.T(x[r], tab)) for r in range(50)) < RTOL
+    plan.close()
+
+
+def test_T_unitary_n60(pkg, torch_mod):
+    say("FCC T unitary n=60")
+    dims = (60, 60, 60)
+    plan, tab = make(pkg, dims, K)
+    x = torch_mod.as_tensor(crand(2, plan.n), device="cuda")
+    y = plan.T(x)
+    assert abs(float(y.norm() / x.norm()) - 1) < 1e-13
+    assert float((plan.TH(y) - x).norm() / x.norm()) < 1e-13
+    xr = x[0].cpu().numpy()
+    assert rel(y[0].cpu().numpy(), fc.T(xr, tab)) < RTOL
+    plan.close()
+
+
+@pytest.mark.parametrize("dims", [(4, 6, 6), (6, 3, 4)])
+def test_sigma_and_lambda_maps(pkg, torch_mod, dims):
+    say(f"FCC Sigma_r / Pr / Qr / Prs / Qrs dims={dims}")
+    plan, tab = make(pkg, dims, K)
+    svd = fc.svd_curl(tab)
+    n = plan.n
+    assert rel(plan.Sigma_r().cpu().numpy(), svd["Sigma_r"]) < RTOL
+    T = lambda v: np.concatenate([fc.T(v[i * n:(i + 1) * n], tab) for i in range(v.size // n)])  # noqa: E731
+    TH = lambda v: np.concatenate([fc.TH(v[i * n:(i + 1) * n], tab) for i in range(v.size // n)])  # noqa: E731
+    u = crand(16 * n)
+    w = crand(24 * n)
+    t = torch_mod
+    assert rel(plan.Pr(t.as_tensor(u)).cpu().numpy(), T(svd["Lambda_Pr"] @ u)) < RTOL
+    assert rel(plan.Qr(t.as_tensor(u)).cpu().numpy(), T(svd["Lambda_Qr"] @ u)) < RTOL
+    assert rel(plan.Prs(t.as_tensor(w)).cpu().numpy(), svd["Lambda_Pr"].conj().T @ TH(w)) < RTOL
+    assert rel(plan.Qrs(t.as_tensor(w)).cpu().numpy(), svd["Lambda_Qr"].conj().T @ TH(w)) < RTOL
+    plan.close()
+
+
+def test_nfgep_and_eigenvector(pkg, torch_mod):
+    say("FCC NFGEP matvec / Compute_eigenvector")
+    dims = (4, 6, 6)
+    plan, tab = make(pkg, dims, K)
+    svd = fc.svd_curl(tab)
+    n = plan.n
+    x = crand(32 * n)
+    xi, zeta, phi = crand(24 * n), crand(24 * n), crand(24 * n) + 3
+    t = torch_mod
+    y = plan.mtx_prod_vec_A_NFGEP(t.as_tensor(x), t.as_tensor(xi), t.as_tensor(zeta), t.as_tensor(phi))
+    assert rel(y.cpu().numpy(), fc.nfgep_A_matvec(x, tab, svd, xi, zeta, phi)) < RTOL
+    y = plan.mtx_prod_vec_A_NFGEP(t.as_tensor(x), 0.3 - 0.2j, 0.3 + 0.2j, 2.5)
+    assert rel(y.cpu().numpy(), fc.nfgep_A_matvec(x, tab, svd, 0.3 - 0.2j, 0.3 + 0.2j, 2.5)) < RTOL
+    ev = plan.Compute_eigenvector(t.as_tensor(x), t.as_tensor(xi), t.as_tensor(zeta), t.as_tensor(phi))
+    assert rel(ev.cpu().numpy(), fc.compute_eigenvector(x, tab, svd, xi, zeta, phi)) < RTOL
+    plan.close()
+
+
+def test_material_operators(pkg, torch_mod):
+    say("FCC mtx_Eps_Mu_Zeta_Xi / GEP B")
+    dims = (4, 6, 6)
+    plan, _ = make(pkg, dims, K)
+    n = plan.n
+    edge = [rng.random(n) < 0.4 for _ in range(4)]
+    face = [np.nonzero(rng.random(n) < 0.3)[0] for _ in range(4)]     # index arrays
+    face_b = [np.isin(np.arange(n), f) for f in face]
+    tin = [crand(3, 3) for _ in range(4)]
+    tair = [crand(3, 3) for _ in range(4)]
+    plan.mtx_Eps_Mu_Zeta_Xi(edge, face, tin, tair)
+    m = fc.eps_mu_zeta_xi(n, edge, face_b, tin, tair)
+    t = torch_mod
+    x = crand(12 * n)
+    for name in pkg.FccPlan.PARAMS:
+        assert rel(plan.param_apply(name, t.as_tensor(x)).cpu().numpy(), m[name] @ x) < RTOL, name
+    x = crand(48 * n)
+    assert rel(plan.GEP_B(t.as_tensor(x)).cpu().numpy(), fc.gep_B(m) @ x) < RTOL
+    plan.close()
