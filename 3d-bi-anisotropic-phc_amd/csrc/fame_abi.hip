@@ -85,6 +85,9 @@ struct fame_plan {
   double2* d_w1 = nullptr;    // [nb][6N]
   double2* d_w2 = nullptr;    // [nb][6N]
   double2* d_part = nullptr;  // reduction partials
+  double2* d_part2 = nullptr; // fused-CG p^H q partials from the MIX_OUT pass
+  bool cg_fused = true;       // CG vector updates fused into fncB (specialised z kernels only)
+  int cg_seq = 0;             // > 0: fused CG runs chunks of this many items to convergence one after another
   double2* d_red = nullptr;   // [nb] reduced values
   int nblk_red = 0;
   // CG scratch
@@ -252,23 +255,30 @@ int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long lon
 
 template <int N, int A, int B, int kZTP, int NT>
 int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs,
-                  int nslot, int grp_base, double sc) {
+                  int nslot, int grp_base, double sc, const CgFuse* cf) {
   const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->cnb);
   const size_t lds = (size_t)(3 * N * kZTP + N) * sizeof(double2);
+  const CgFuse c0{nullptr, nullptr, nullptr, nullptr, 0};
   if (dir > 0) {
-    if (mode == ZM_MIX_IN)
+    if (mode == ZM_MIX_IN && cf)
+      hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT, 1>), grid, dim3(NT), lds, P->stream, P->geo, in,
+                         ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, *cf);
+    else if (mode == ZM_MIX_IN)
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
-                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
+                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, c0);
     else
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
-                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
+                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, c0);
   } else {
-    if (mode == ZM_MIX_OUT)
+    if (mode == ZM_MIX_OUT && cf)
+      hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT, 1>), grid, dim3(NT), lds, P->stream, P->geo, in,
+                         ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, *cf);
+    else if (mode == ZM_MIX_OUT)
       hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in,
-                         ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
+                         ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, c0);
     else
       hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
-                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base);
+                         out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, c0);
   }
   CHECK_LAUNCH();
   return FAME_OK;
@@ -291,6 +301,8 @@ int z_attrs() {
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_PLAIN, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_PLAIN, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_MIX_IN, TP, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, TP, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   return FAME_OK;
 }
 
@@ -312,19 +324,21 @@ int set_fast_attrs(int n) {
 }
 
 // ---- launch helpers -------------------------------------------------------
+// cf (fast z kernels only): fuse the CG updates into the MIX_IN / MIX_OUT pass
 int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs, int nslot,
-          int grp_base) {
+          int grp_base, const CgFuse* cf = nullptr) {
   const double sc = 1.0 / std::sqrt((double)P->n3);
   if (P->fast && use_fast_z(P)) {
 #define X(N, A, B)                                                                                       \
     if (P->n3 == N) {                                                                                      \
-      if (P->zv == 1) return z_fast_launch<N, A, B, 8, 128>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc); \
-      if (P->zv == 2) return z_fast_launch<N, A, B, 32, 512>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc); \
-      return z_fast_launch<N, A, B, 16, 256>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc);        \
+      if (P->zv == 1) return z_fast_launch<N, A, B, 8, 128>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf); \
+      if (P->zv == 2) return z_fast_launch<N, A, B, 32, 512>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf); \
+      return z_fast_launch<N, A, B, 16, 256>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf);        \
     }
     FAME_FAST_SIZES(X)
 #undef X
   }
+  if (cf) return set_err(FAME_ERR_UNSUPPORTED, "fused CG needs the specialised z kernels");
   const dim3 grid((P->geo.n12 + kZTile - 1) / kZTile, nslot, P->cnb);
   const size_t lds = zpass_lds(P);
   if (dir > 0) {
@@ -421,7 +435,8 @@ int twist(fame_plan* P, const double2* in, double2* out, int ncomp, int conj_twi
 }
 
 // One fncB over batch items [b0, b0 + nbc): z-pass + Pi mix, xy-pass, material, xy-pass, z-pass + Pi^H.
-int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y) {
+// With cf (fused CG), x is the CG residual r and the transformed vector is p <- r + beta p.
+int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, const CgFuse* cf = nullptr) {
   const long long N = P->N;
   struct Window {  // restores the full-batch window on every exit path
     fame_plan* P;
@@ -431,9 +446,19 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y) {
   P->cnb = nbc;
   x += (size_t)b0 * 4 * N;
   y += (size_t)b0 * 4 * N;
+  CgFuse wcf{};
+  const CgFuse* pcf = nullptr;
+  if (cf) {
+    wcf = *cf;
+    wcf.st = cf->st + b0;
+    wcf.x = cf->x + (size_t)b0 * 4 * N;
+    wcf.p = cf->p + (size_t)b0 * 4 * N;
+    wcf.part = cf->part + (size_t)b0 * cf->nparts;
+    pcf = &wcf;
+  }
   const bool pr = P->prof;
   if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
-  RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0));
+  RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0, pcf));
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
   RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
@@ -441,7 +466,7 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y) {
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
   RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
-  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0));
+  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, pcf));
   if (pr) {
     HIP_TRY(hipEventRecord(P->ev[5], P->stream));
     HIP_TRY(hipEventSynchronize(P->ev[5]));
@@ -543,7 +568,101 @@ int copy(fame_plan* P, const double2* x, double2* y, long long len) {
   return FAME_OK;
 }
 
+int cg_finish(fame_plan* P, std::vector<CgState>& hs, int* iters, double* relres) {
+  const int nb = P->vnb;
+  HIP_TRY(hipMemcpyAsync(hs.data(), P->d_cg, nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  bool all = true;
+  for (int i = 0; i < nb; ++i) {
+    if (iters) iters[i] = (int)hs[i].iters;
+    if (relres) relres[i] = hs[i].bnorm2 > 0 ? std::sqrt(hs[i].rho / hs[i].bnorm2) : 0.0;
+    if (hs[i].active != 0.0) all = false;
+  }
+  if (!all) return set_err(FAME_ERR_NOT_CONVERGED, "fncinvB: CG reached maxit before tol");
+  return FAME_OK;
+}
+
+int zfast_tp(const fame_plan* P) { return P->zv == 1 ? 8 : (P->zv == 2 ? 32 : 16); }
+
+// CG with its vector updates fused into fncB (same iterates as the unfused loop below up
+// to rounding): per chunk of items, fncB(p) with p <- r + beta p and the deferred
+// x <- x + alpha p done in the first pass and p^H q summed in the last, then alpha,
+// r <- r - alpha q with ||r||^2, and beta -- all while the chunk's vectors are recent in
+// the last-level cache.  x receives its last alpha p after the loop.
+int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int* iters, double* relres) {
+  const long long L = 4 * P->N;
+  const long long tot = L * P->vnb;
+  const int nb = P->vnb;
+  const int nparts = 2 * ((P->geo.n12 + zfast_tp(P) - 1) / zfast_tp(P));
+  if (!P->d_part2) HIP_TRY(hipMalloc(&P->d_part2, (size_t)P->nb * 2 * ((P->geo.n12 + 7) / 8) * sizeof(double2)));
+  RET_IF(fill_zero(P, x, tot));
+  RET_IF(copy(P, b, P->d_r, tot));
+  RET_IF(fill_zero(P, P->d_p, tot));
+  RET_IF(do_dot(P, b, L, b, L, L, nb, P->d_red));
+  hipLaunchKernelGGL(k_cg_init, dim3((nb + 63) / 64), dim3(64), 0, P->stream, P->d_red, P->d_cg, nb, tol);
+  CHECK_LAUNCH();
+  const CgFuse cf{P->d_cg, x, P->d_p, P->d_part2, nparts};
+  const int nblk = std::min<int>(P->nblk_red, (int)grid1d(L));
+  std::vector<CgState> hs(nb);
+  // one iteration on the chunk [b0, b0 + nbc)
+  auto step = [&](int b0, int nbc) -> int {
+    RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_q, &cf));
+    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts, nparts,
+                       P->d_cg + b0);
+    hipLaunchKernelGGL(k_cg_update_r, dim3(nblk, nbc), dim3(kBlk), 0, P->stream, P->d_r + (size_t)b0 * L,
+                       P->d_q + (size_t)b0 * L, L, P->d_cg + b0, P->d_part + (size_t)b0 * nblk);
+    hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part + (size_t)b0 * nblk, nblk,
+                       P->d_cg + b0);
+    CHECK_LAUNCH();
+    return FAME_OK;
+  };
+  // all items of [b0, b0 + nbc) converged?
+  auto finished = [&](int b0, int nbc, bool& done) -> int {
+    HIP_TRY(hipMemcpyAsync(hs.data() + b0, P->d_cg + b0, nbc * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    done = true;
+    for (int i = b0; i < b0 + nbc; ++i)
+      if (hs[i].active != 0.0) done = false;
+    return FAME_OK;
+  };
+  if (P->cg_seq > 0) {
+    // chunk by chunk to convergence: a chunk's vectors and fncB intermediates stay in the
+    // last-level cache for all its iterations
+    const int c = std::min(P->cg_seq, nb);
+    for (int b0 = 0; b0 < nb; b0 += c) {
+      const int nbc = std::min(c, nb - b0);
+      bool done = false;
+      for (int it = 0; it < maxit && !done;) {
+        RET_IF(step(b0, nbc));
+        ++it;
+        if (it % 4 == 0 || it == maxit) RET_IF(finished(b0, nbc, done));
+      }
+    }
+  } else {
+    // all items in lock step, chunk by chunk within an iteration
+    const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
+    bool done = false;
+    for (int it = 0; it < maxit && !done;) {
+      for (int b0 = 0; b0 < nb; b0 += c) RET_IF(step(b0, std::min(c, nb - b0)));
+      if (P->prof) P->prof_calls += 1;
+      ++it;
+      if (it % 4 == 0 || it == maxit) RET_IF(finished(0, nb, done));
+    }
+  }
+  hipLaunchKernelGGL(k_cg_xfin, dim3(grid1d(L), nb), dim3(kBlk), 0, P->stream, x, P->d_p, L, P->d_cg);
+  CHECK_LAUNCH();
+  return cg_finish(P, hs, iters, relres);
+}
+
+int do_cg_plain(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int* iters, double* relres);
+
 int do_cg(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int* iters, double* relres) {
+  if (P->cg_fused && P->fast && use_fast_z(P)) return do_cg_fused(P, b, x, tol, maxit, iters, relres);
+  return do_cg_plain(P, b, x, tol, maxit, iters, relres);
+}
+
+// CG as in MATLAB pcg without preconditioner: separate dot, update and direction kernels.
+int do_cg_plain(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int* iters, double* relres) {
   const long long L = 4 * P->N;
   const long long tot = L * P->vnb;
   const int nb = P->vnb;
@@ -576,16 +695,7 @@ int do_cg(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int
         if (hs[i].active != 0.0) done = false;
     }
   }
-  HIP_TRY(hipMemcpyAsync(hs.data(), P->d_cg, nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
-  HIP_TRY(hipStreamSynchronize(P->stream));
-  bool all = true;
-  for (int i = 0; i < nb; ++i) {
-    if (iters) iters[i] = (int)hs[i].iters;
-    if (relres) relres[i] = hs[i].bnorm2 > 0 ? std::sqrt(hs[i].rho / hs[i].bnorm2) : 0.0;
-    if (hs[i].active != 0.0) all = false;
-  }
-  if (!all) return set_err(FAME_ERR_NOT_CONVERGED, "fncinvB: CG reached maxit before tol");
-  return FAME_OK;
+  return cg_finish(P, hs, iters, relres);
 }
 
 }  // namespace
@@ -724,7 +834,7 @@ int fame_plan_destroy(fame_plan* P) {
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
-                  P->d_idx};
+                  P->d_idx, P->d_part2};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
@@ -835,6 +945,14 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
 int fame_plan_set_generic(fame_plan* P, int generic) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   P->fast = (generic == 0);
+  return FAME_OK;
+}
+
+int fame_plan_set_cg_fused(fame_plan* P, int fused) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  if (fused < 0) return set_err(FAME_ERR_ARG, "fused must be >= 0");
+  P->cg_fused = (fused != 0);
+  P->cg_seq = fused > 1 ? fused - 1 : 0;
   return FAME_OK;
 }
 

@@ -170,11 +170,25 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
 // z-pencil pass (n3 = N3 = A*B), TP consecutive pencils per work-group.
 // LDS: [3][N3][TP] (pencil index fastest) + twiddles (N3).
 // ---------------------------------------------------------------------------
-template <int N3, int A, int B, int DIR, int MODE, int TP, int NT>
+// CG = 1 fuses the conjugate-gradient vector updates into the first and last pass of fncB:
+//   MIX_IN  reads r (the input), p and x at each point, forms p <- r + beta p and the
+//           deferred x <- x + alpha p_old, writes p and x back and transforms p;
+//   MIX_OUT accumulates conj(p) . q over its outputs q and leaves one partial sum per
+//           (item, slot, work-group) in cf.part[item * cf.nparts + slot * gridDim.x + blockIdx.x].
+struct CgFuse {
+  const CgState* st;
+  double2* x;
+  double2* p;
+  double2* part;
+  int nparts;
+};
+
+template <int N3, int A, int B, int DIR, int MODE, int TP, int NT, int CG = 0>
 __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__ in, long long in_bstride,
                                               double2* __restrict__ out, long long out_bstride,
                                               const double2* __restrict__ lam, int lam_bstride,
-                                              const double2* __restrict__ twg, double scale, int grp_base) {
+                                              const double2* __restrict__ twg, double scale, int grp_base,
+                                              CgFuse cf) {
   static_assert(A * B == N3, "k_zfast: N3 = A*B");
   extern __shared__ double2 sm[];
   double2* tw = sm + 3 * N3 * TP;
@@ -192,13 +206,32 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
   if (MODE == ZM_MIX_IN) {
     // phase 0: load + Pi mix in natural order
     const double2* __restrict__ xin = in + b * in_bstride + (long long)(2 * slot) * N;
+    double2* __restrict__ pf = nullptr;
+    double2* __restrict__ xf = nullptr;
+    double cal = 0, cbe = 0;
+    if (CG) {
+      pf = cf.p + b * in_bstride + (long long)(2 * slot) * N;
+      xf = cf.x + b * in_bstride + (long long)(2 * slot) * N;
+      cal = cf.st[b].alpha;
+      cbe = cf.st[b].beta;
+    }
     for (int it = threadIdx.x; it < N3 * TP; it += NT) {
       const int k3 = it / TP, pp = it - (it / TP) * TP;
       const int q = q0 + pp;
       double2 v0 = make_double2(0, 0), v1 = v0, v2 = v0;
       if (q < n12) {
         const long long P = q + (long long)n12 * k3;
-        const double2 u1 = xin[P], u2 = xin[N + P];
+        double2 u1 = xin[P], u2 = xin[N + P];
+        if (CG) {
+          const double2 o1 = pf[P], o2 = pf[N + P];
+          const double2 x1 = xf[P], x2 = xf[N + P];
+          u1 = make_double2(fma(cbe, o1.x, u1.x), fma(cbe, o1.y, u1.y));
+          u2 = make_double2(fma(cbe, o2.x, u2.x), fma(cbe, o2.y, u2.y));
+          xf[P] = make_double2(fma(cal, o1.x, x1.x), fma(cal, o1.y, x1.y));
+          xf[N + P] = make_double2(fma(cal, o2.x, x2.x), fma(cal, o2.y, x2.y));
+          pf[P] = u1;
+          pf[N + P] = u2;
+        }
         const int k2 = q / g.n1, k1 = q - k2 * g.n1;
         double2 p1[3], p2[3];
         pi_vectors(lb[k1], lb[g.n1 + k2], lb[g.n1 + g.n2 + k3], p1, p2);
@@ -277,6 +310,8 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
     }
     __syncthreads();
     double2* __restrict__ yout = out + b * out_bstride + (long long)(2 * slot) * N;
+    const double2* __restrict__ pq = CG ? cf.p + b * out_bstride + (long long)(2 * slot) * N : nullptr;
+    double2 dacc = make_double2(0, 0);
     for (int it = threadIdx.x; it < N3 * TP; it += NT) {
       const int k3 = it / TP, pp = it - (it / TP) * TP;
       const int q = q0 + pp;
@@ -298,6 +333,11 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
       const long long P = q + (long long)n12 * k3;
       yout[P] = o1;
       yout[N + P] = o2;
+      if (CG) dacc = cfmac(cfmac(dacc, pq[P], o1), pq[N + P], o2);
+    }
+    if (CG) {
+      const double2 s = block_sum(dacc);
+      if (threadIdx.x == 0) cf.part[(long long)b * cf.nparts + slot * gridDim.x + blockIdx.x] = s;
     }
   } else {
     // phase B straight to HBM: item (c, k, pp); output z = k + A q

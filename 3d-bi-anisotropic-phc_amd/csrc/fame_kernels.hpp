@@ -551,6 +551,56 @@ __global__ void k_cg_p(double2* __restrict__ p, const double2* __restrict__ r, l
   }
 }
 
+// Fused-CG scalar and vector steps (the p / x updates and the p^H q partial sums run
+// inside the first and last pass of fncB):
+// alpha = rho / Re(p^H q) from the partial sums of the MIX_OUT pass
+__global__ void __launch_bounds__(256) k_cg_alpha2(const double2* __restrict__ part, int nparts,
+                                                   CgState* __restrict__ st) {
+  const int b = blockIdx.x;
+  double2 acc = make_double2(0, 0);
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = cadd(acc, part[(long long)b * nparts + i]);
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) {
+    CgState s = st[b];
+    s.alpha = (s.active != 0.0) ? s.rho / acc.x : 0.0;
+    st[b] = s;
+  }
+}
+
+// r -= alpha q ; partial ||r||^2
+__global__ void __launch_bounds__(256) k_cg_update_r(double2* __restrict__ r, const double2* __restrict__ q,
+                                                     long long len, const CgState* __restrict__ st,
+                                                     double2* __restrict__ part) {
+  const int b = blockIdx.y;
+  const double a = st[b].alpha;
+  const long long off = b * len;
+  double acc = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x) {
+    const double2 qv = q[off + i];
+    double2 rv = r[off + i];
+    rv.x = fma(-a, qv.x, rv.x);
+    rv.y = fma(-a, qv.y, rv.y);
+    r[off + i] = rv;
+    acc = fma(rv.x, rv.x, fma(rv.y, rv.y, acc));
+  }
+  double2 sm = block_sum(make_double2(acc, 0));
+  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = sm;
+}
+
+// the deferred last update x += alpha p
+__global__ void k_cg_xfin(double2* __restrict__ x, const double2* __restrict__ p, long long len,
+                          const CgState* __restrict__ st) {
+  const int b = blockIdx.y;
+  const double a = st[b].alpha;
+  if (a == 0.0) return;
+  const long long off = b * len;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x) {
+    const double2 pv = p[off + i];
+    double2 xv = x[off + i];
+    x[off + i] = make_double2(fma(a, pv.x, xv.x), fma(a, pv.y, xv.y));
+  }
+}
+
 // out(:, 0:k) = Q(:, 0:m) * U(0:m, 0:k) for real U (column-major, leading dim ldu), per batch item.
 // Q and out are [b][col][len]; out must not alias Q.
 __global__ void k_tsgemm(const double2* __restrict__ Q, long long qbs, long long len, const double* __restrict__ U,

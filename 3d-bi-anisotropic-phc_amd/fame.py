@@ -84,6 +84,7 @@ def load_library(path: str = LIB_PATH):
         "fame_block_gemv": ([vp, vp, i, ctypes.c_longlong, vp, d, vp], i),
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
+        "fame_plan_set_cg_fused": ([vp, i], i),
         "fame_plan_set_chunk": ([vp, i], i),
         "fame_plan_get_chunk": ([vp], i),
         "fame_plan_autotune": ([vp, pd], i),
@@ -122,7 +123,7 @@ def exported_symbols():
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv",
-        "fame_plan_profile", "fame_plan_stage_times",
+        "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
         "fame_fcc_create", "fame_fcc_destroy", "fame_fcc_set_stream", "fame_fcc_sync", "fame_fcc_T", "fame_fcc_TH",
@@ -237,6 +238,13 @@ class Plan:
         _check(self.lib.fame_plan_set_generic(self.h, int(bool(generic))))
 
     STAGES = ("zpass_mix_in", "xypass_inv", "material", "xypass_fwd", "zpass_mix_out")
+
+    def set_cg_fused(self, fused=True, seq_chunk=0):
+        """CG schedule for fncinvB and Lanczos (same iterates up to rounding): fused lock step
+        (default), fused with chunks of ``seq_chunk`` k-points solved one after another, or the
+        textbook loop (fused=False)."""
+        v = 0 if not fused else (1 + int(seq_chunk) if seq_chunk > 0 else 1)
+        _check(self.lib.fame_plan_set_cg_fused(self.h, v))
 
     def profile(self, enable=True):
         """Per-kernel HIP-event timing of fncB (measurement aid)."""

@@ -99,6 +99,15 @@ int fame_plan_get_variant(fame_plan* plan, int* xy_variant, int* z_variant);
  * that cover every supported size (used to cross-check the two). */
 int fame_plan_set_generic(fame_plan* plan, int generic);
 
+/* CG schedule (no reference counterpart; same iterates up to rounding): fused = 1
+ * (default) folds the CG vector updates and the p^H q reduction into the first and
+ * last pass of fncB and advances all k-points in lock step, chunk by chunk;
+ * fused = 1 + c runs chunks of c k-points to convergence one after another (their
+ * working set stays in the last-level cache); 0 runs the textbook loop with separate
+ * dot / update / direction kernels.  The fused forms need the specialised z kernels
+ * and fall back to the textbook loop otherwise. */
+int fame_plan_set_cg_fused(fame_plan* plan, int fused);
+
 /* Measurement aid (no reference counterpart): with enable != 0 every fame_fncB
  * records HIP events on the plan's stream between its five kernels
  * (z-pass+Pi mix, xy-pass inverse, material stencil, xy-pass forward,

@@ -185,6 +185,42 @@ def test_fncinvB(pkg, fo, torch_mod):
     plan.close()
 
 
+@pytest.mark.parametrize("n,chunk", [(8, 0), (12, 1), (60, 0)])
+def test_fused_cg_matches_textbook_cg(pkg, torch_mod, n, chunk):
+    """The fused CG schedule (updates inside fncB, chunk by chunk) gives the textbook CG iterates."""
+    say(f"fused vs textbook CG n={n} chunk={chunk}")
+    dims = (n, n, n)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE)
+                       for k in ([0.5, 0.3, 0.1], [0.2, 0.0, 0.9], [0.7, 0.7, 0.1])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    plan.set_chunk(chunk)
+    x = torch_mod.as_tensor(crand(3, 4 * plan.N))
+    x[1] = 0  # an item with b = 0 stays at x = 0
+    yf, itf, rf = plan.fncinvB(x, tol=1e-12, maxit=3000)
+    plan.set_cg_fused(False)
+    yt, itt, rt = plan.fncinvB(x, tol=1e-12, maxit=3000)
+    yf, yt = yf.cpu().numpy(), yt.cpu().numpy()
+    assert not yf[1].any() and itf[1] == 0
+    for b in (0, 2):
+        assert abs(int(itf[b]) - int(itt[b])) <= 1
+        assert rf[b] <= 1e-12 and rt[b] <= 1e-12
+        assert rel(yf[b], yt[b]) < 1e-10
+    if n == 8:
+        for b in (0, 2):
+            P = oracle_problem(pkg_fo(), dims, thetas[b], table, ids)
+            xb = x[b].cpu().numpy()
+            assert np.linalg.norm(P.fncB(yf[b]) - xb) <= 2e-12 * np.linalg.norm(xb)
+    plan.close()
+
+
+def pkg_fo():
+    from oracle import fame_oracle
+
+    return fame_oracle
+
+
 def test_fncinvB_maxit_reports_not_converged(pkg, torch_mod):
     say("fncinvB maxit")
     dims = (8, 8, 8)
