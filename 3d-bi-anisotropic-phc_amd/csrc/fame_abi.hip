@@ -88,6 +88,13 @@ struct fame_plan {
   double2* d_part2 = nullptr; // fused-CG p^H q partials from the MIX_OUT pass
   bool cg_fused = true;       // CG vector updates fused into fncB (specialised z kernels only)
   int cg_seq = 0;             // > 0: fused CG runs chunks of this many items to convergence one after another
+  // per-vertex bi-isotropic material (eps_v, gamma_v at every vertex; replaces the id table)
+  bool pv = false;
+  double pv_G[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double pv_mu = 1.0;
+  double* d_eps = nullptr;
+  double* d_gam = nullptr;
+  double2* d_pvw = nullptr;   // vertex products, [nb][6N]
   double2* d_red = nullptr;   // [nb] reduced values
   int nblk_red = 0;
   // CG scratch
@@ -408,7 +415,10 @@ void material_tile(int n1, int n2, int n3, int nbc, int& TX, int& TY, int& TZ) {
   TZ = (int)((n3 + nzc - 1) / nzc);
 }
 
+int pv_material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs);
+
 int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs) {
+  if (P->pv) return pv_material(P, in, ibs, out, obs);
   if (P->fast) {
     int TX, TY, TZ;
     material_tile(P->n1, P->n2, P->n3, P->cnb, TX, TY, TZ);
@@ -834,7 +844,7 @@ int fame_plan_destroy(fame_plan* P) {
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
-                  P->d_idx, P->d_part2};
+                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
@@ -1057,3 +1067,4 @@ int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit,
 #include "fame_lanczos.inc"
 #include "fame_minres.inc"
 #include "fame_fcc.inc"
+#include "fame_vertex.inc"

@@ -91,6 +91,8 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_set_variant": ([vp, i, i], i),
         "fame_plan_get_variant": ([vp, pi_, pi_], i),
         "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
+        "fame_plan_set_vertex_material": ([vp, pd, pd, d, pd], i),
+        "fame_material_gradient": ([vp, i, vp, vp], i),
         "fame_fcc_create": ([ctypes.POINTER(vp), i, i, i, pd, pd, pd, pd, pd], i),
         "fame_fcc_destroy": ([vp], i),
         "fame_fcc_set_stream": ([vp, vp], i),
@@ -126,6 +128,7 @@ def exported_symbols():
         "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
+        "fame_plan_set_vertex_material", "fame_material_gradient",
         "fame_fcc_create", "fame_fcc_destroy", "fame_fcc_set_stream", "fame_fcc_sync", "fame_fcc_T", "fame_fcc_TH",
         "fame_fcc_Sigma_r", "fame_fcc_Pr", "fame_fcc_Qr", "fame_fcc_Prs", "fame_fcc_Qrs", "fame_fcc_nfgep_A",
         "fame_fcc_eigenvector", "fame_fcc_set_material", "fame_fcc_param_apply", "fame_fcc_gep_B",
@@ -377,6 +380,31 @@ class Plan:
         _check(self.lib.fame_eigen_restoration(self.h, int(nvec), ctypes.c_void_p(ev.data_ptr()),
                                                ctypes.c_void_p(out.data_ptr())))
         return out
+
+    def set_vertex_material(self, eps, gamma, mu=1.0, lattice=None):
+        """Per-vertex bi-isotropic material (eps_v, gamma_v at every vertex, x fastest) in place of
+        the id table; ``lattice`` has the lattice vectors as columns (default simple cubic)."""
+        e = np.ascontiguousarray(np.asarray(eps, dtype=np.float64).reshape(-1))
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gamma, dtype=np.float64), e.shape).copy())
+        if e.size != self.N:
+            raise ValueError(f"expected {self.N} vertex values")
+        A = np.ascontiguousarray(np.eye(3) if lattice is None else np.asarray(lattice, dtype=np.float64).reshape(3, 3))
+        _check(self.lib.fame_plan_set_vertex_material(self.h, _dptr(e), _dptr(g), float(mu), _dptr(A)))
+
+    def material_gradient(self, ev):
+        """d(x^H fncB x)/d eps_v, d/d gamma_v and the vertex terms for NFSEP vectors ev
+        ([nb,] nvec, 4N); returns a device tensor ([nb,] nvec, 3, N)."""
+        t = self.torch
+        x = ev.to(device=self.device, dtype=t.complex128).contiguous()
+        squeeze = x.dim() == 2
+        if squeeze:
+            x = x.unsqueeze(0)
+        nvec = x.shape[1]
+        if x.shape[0] != self.nb or x.shape[2] != 4 * self.N:
+            raise ValueError("ev must be (nb, nvec, 4N)")
+        g = t.empty((self.nb, nvec, 3, self.N), dtype=t.float64, device=self.device)
+        _check(self.lib.fame_material_gradient(self.h, nvec, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(g.data_ptr())))
+        return g[0] if squeeze else g
 
     def Sigma_r(self):
         """Sigma_r = [sqrt(Lambda_q); sqrt(Lambda_q)] per item (SVD_Matrix_curl.m:74), on the host."""

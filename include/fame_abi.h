@@ -187,6 +187,23 @@ int fame_eigen_restoration(fame_plan* plan, int nvec, const void* ev, void* e_h)
  * written to device memory (nb complex). */
 int fame_dot(fame_plan* plan, const void* x, const void* y, long long len, void* out);
 
+/* Per-vertex bi-isotropic material, replacing the id table of fame_plan_create:
+ * eps_v I, mu I, xi = i gamma_v I, zeta = -i gamma_v I at every vertex (host arrays
+ * eps[N], gamma[N], x fastest), lattice vectors as the columns of the row-major 3x3
+ * `lattice`.  The slots are built as in Physical_settings_woodpile.m:91-96,184-194 and
+ * Public/Material_Matrix_Generate_Mult_Point.m:28-31 (per-vertex A{i,j} instead of
+ * inside/outside values).  Needs eps*mu - gamma^2 > 0 everywhere. */
+int fame_plan_set_vertex_material(fame_plan* plan, const double* eps, const double* gamma, double mu,
+                                  const double* lattice);
+
+/* Adjoint of the material apply (no reference counterpart: the differentiation the
+ * reference does not have).  For each of nvec NFSEP vectors x (device, [nb][nvec][4N])
+ * writes grad[nb][nvec][3][N] (device, double):
+ *   grad[.][j][0][v] = d(x^H fncB x)/d eps_v,  grad[.][j][1][v] = d(x^H fncB x)/d gamma_v,
+ *   grad[.][j][2][v] = the vertex-v term of x^H fncB x (they sum to x^H fncB x).
+ * For fncB x = w fncA x, dw/dp = grad / (x^H fncA x).  Needs the per-vertex material. */
+int fame_material_gradient(fame_plan* plan, int nvec, const void* ev, void* grad);
+
 /* ---------------------------------------------------------------------------
  * FCC skewed-grid operators (the reference's light_locked_Huang/ FCC path).
  *

@@ -655,6 +655,56 @@ def evaluate_error(ev, ew, e, h, C, fA, fB, mtx_B):
 # --------------------------------------------------------------------------
 
 
+def vertex_material(eps_v, gam_v, mu, A):
+    """Per-vertex (N-slot, M-slot, J-slot) tensors, each (N, 3, 3), of the bi-isotropic media
+    eps_v I, mu I, xi = i gamma_v I, zeta = -i gamma_v I: the inverse 6x6 by a batched general
+    inverse and the covariant form A' X A (Physical_settings_woodpile.m:91-96,184-194), stored
+    per vertex as Material_Matrix_Generate_Mult_Point.m:38-48 stores the A{i,j} diagonals."""
+    C = _bi_isotropic_6x6(eps_v, gam_v, mu)
+    Ci = np.linalg.inv(C)
+    return _slots_cov(Ci, A)
+
+
+def vertex_material_derivative(eps_v, gam_v, mu, A, wrt):
+    """d(slots)/d eps_v (wrt = 'eps') or d/d gamma_v ('gamma') per vertex: -C^-1 (dC) C^-1."""
+    C = _bi_isotropic_6x6(eps_v, gam_v, mu)
+    Ci = np.linalg.inv(C)
+    dC = np.zeros_like(C)
+    I3 = np.eye(3)
+    if wrt == "eps":
+        dC[:, :3, :3] = I3
+    else:
+        dC[:, :3, 3:] = 1j * I3
+        dC[:, 3:, :3] = -1j * I3
+    return _slots_cov(-Ci @ dC @ Ci, A)
+
+
+def _bi_isotropic_6x6(eps_v, gam_v, mu):
+    eps_v = np.asarray(eps_v, float)
+    gam_v = np.broadcast_to(np.asarray(gam_v, float), eps_v.shape)
+    n = eps_v.size
+    C = np.zeros((n, 6, 6), complex)
+    I3 = np.eye(3)
+    C[:, :3, :3] = eps_v[:, None, None] * I3
+    C[:, 3:, 3:] = mu * I3
+    C[:, :3, 3:] = 1j * gam_v[:, None, None] * I3
+    C[:, 3:, :3] = -1j * gam_v[:, None, None] * I3
+    return C
+
+
+def _slots_cov(Ci, A):
+    A = np.asarray(A, float)
+    cov = lambda X: np.einsum("ki,nkl,lj->nij", A, X, A)  # noqa: E731  A' X A per vertex
+    return cov(Ci[:, :3, :3]), cov(Ci[:, 3:, 3:]), cov(Ci[:, :3, 3:])
+
+
+def quadratic_form_derivative(dims, theta, x, dN, dM, dJ, delta=None):
+    """x^H (d fncB) x for the material direction (dN, dM, dJ): fncB is linear in the vertex
+    tensors, so this is fncB assembled from the derivative tensors (the reference gradient)."""
+    P = Problem(dims, theta, dN, dM, dJ, delta)
+    return np.vdot(x, P.fncB(x))
+
+
 class Problem:
     """One (grid, k, material) instance: mtx_A, fnc, mtx_B and the operator handles."""
 

@@ -269,3 +269,27 @@ def test_minres_iterates_are_krylov_least_squares(fo, shifted):
         assert abs(relres - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) < 1e-9
     x, flag, relres, it = fo.minres(lambda v: A @ v, b, 1e-12, 500)
     assert flag == 0 and np.linalg.norm(x - np.linalg.solve(A, b)) <= 1e-10 * np.linalg.norm(x)
+
+
+def test_vertex_material_closed_form(fo):
+    """Per-vertex slots (general batched inverse) = the bi-isotropic closed form n = mu/d,
+    m = eps/d, j = -i gamma/d (d = eps mu - gamma^2) times A'A; derivatives likewise."""
+    rng = np.random.default_rng(8)
+    e = rng.uniform(1, 13, 7)
+    g = rng.uniform(0, 0.9, 7)
+    mu = 1.3
+    A = rng.standard_normal((3, 3))
+    G = A.T @ A
+    N, M, J = fo.vertex_material(e, g, mu, A)
+    d = e * mu - g ** 2
+    assert np.allclose(N, (mu / d)[:, None, None] * G, rtol=1e-13, atol=0)
+    assert np.allclose(M, (e / d)[:, None, None] * G, rtol=1e-13, atol=0)
+    assert np.allclose(J, (-1j * g / d)[:, None, None] * G, rtol=1e-13, atol=0)
+    h = 1e-6
+    for wrt, (ep, gp) in (("eps", (e + h, g)), ("gamma", (e, g + h))):
+        em, gm = (e - h, g) if wrt == "eps" else (e, g - h)
+        dN, dM, dJ = fo.vertex_material_derivative(e, g, mu, A, wrt)
+        Np, Mp, Jp = fo.vertex_material(ep, gp, mu, A)
+        Nm, Mm, Jm = fo.vertex_material(em, gm, mu, A)
+        for D, Pp, Pm in ((dN, Np, Nm), (dM, Mp, Mm), (dJ, Jp, Jm)):
+            assert np.allclose(D, (Pp - Pm) / (2 * h), rtol=1e-7, atol=1e-9)
