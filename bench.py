@@ -63,9 +63,281 @@ def cpu_baseline(n, seconds):
         x /= np.linalg.norm(x)
         cnt += 1
     dt = time.perf_counter() - t0
-    return {"value": cnt / dt, "unit": "matvecs/s", "cores": os.cpu_count(), "kind": "port",
+    return {"value": cnt / dt, "unit": "matvecs/s", "cores": fo.FFT_WORKERS, "kind": "port",
             "sample": f"oracle fncB (CPU restatement of the reference algorithm, not MATLAB) at n={n}, one k-point, "
-                      f"{cnt} calls in {dt:.1f} s; scipy.fft workers={os.cpu_count()}, CSR SpMV single-threaded"}
+                      f"{cnt} calls in {dt:.1f} s; scipy.fft workers={fo.FFT_WORKERS}, CSR SpMV single-threaded"}
+
+
+def _time_op(torch, fn, reps, warm=2):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def op_rates(pkg, torch, n, nk=40, reps=20):
+    """Throughput of every operator of the path at n (complex128), batched over nk k-points
+    where the operator takes a batch, with algorithmic HBM bytes and the fraction of peak."""
+    S = pkg.structures
+    dims = (n, n, n)
+    N = n ** 3
+    ks = S.sc_band_path()
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in ks[:nk]])
+    table = S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE)
+    ids = S.woodpile_sc_ids(dims)
+    pl = pkg.Plan(dims, thetas, table, ids)
+    dev = pl.device
+    g = torch.Generator(device=dev).manual_seed(7)
+    x4 = torch.randn(nk, 4 * N, dtype=torch.complex128, device=dev, generator=g)
+    x3 = torch.randn(nk, 3 * N, dtype=torch.complex128, device=dev, generator=g)
+    x2 = torch.randn(nk, 2 * N, dtype=torch.complex128, device=dev, generator=g)
+    out = {}
+
+    def rec(name, ms, items, bytes_per_item, unit_items="applications"):
+        gbs = bytes_per_item * items / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "per_s": items / (ms * 1e-3), "unit": f"{unit_items}/s", "algo_GBs": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS, "algo_bytes_per_item": bytes_per_item, "items_per_call": items}
+
+    rec("fncA", _time_op(torch, lambda: pl.fncA(x4), reps), nk, 128.0 * N)
+    rec("MtxVecMul_FFT_notranspose", _time_op(torch, lambda: pl.MtxVecMul_FFT(x3, "notranspose"), reps), nk, 96.0 * N)
+    rec("MtxVecMul_FFT_transpose", _time_op(torch, lambda: pl.MtxVecMul_FFT(x3, "transpose"), reps), nk, 96.0 * N)
+    rec("Pr", _time_op(torch, lambda: pl.Pr(x2), reps), nk, 80.0 * N)
+    rec("Prs", _time_op(torch, lambda: pl.Prs(x3), reps), nk, 80.0 * N)
+    # per-vertex material (eps_v, gamma_v: +16 B per vertex) and its adjoint kernel
+    r = np.random.default_rng(5)
+    pl.set_vertex_material(np.where(ids == 1, r.uniform(1, 13, N), 1.0), np.where(ids == 1, 0.2, 0.0))
+    rec("fncB_vertex_material", _time_op(torch, lambda: pl.fncB(x4), max(3, reps // 4)), nk, (512.0 + 16.0) * N)
+    ev = x4[:, None, :].contiguous()
+    # algorithmic: read the eigenvector (4 words) and eps/gamma (16 B), write 3 doubles per vertex
+    rec("material_gradient", _time_op(torch, lambda: pl.material_gradient(ev), max(3, reps // 4)), nk,
+        (64.0 + 16.0 + 24.0) * N, "eigenvectors")
+    pl.close()
+    del x4, x3, x2, ev
+    # FCC skewed grid at the same n (one k, vectors batched)
+    from oracle import fcc_oracle  # noqa: F401  (lattice constants only)
+    lat = (np.array([1.0, 0.0, 0.0]), np.array([0.5, np.sqrt(3) / 2, 0.0]), np.array([0.5, np.sqrt(3) / 6, np.sqrt(6) / 3]))
+    delta = np.array([1 / n, np.sqrt(3) / 2 / n, np.sqrt(6) / 3 / n])
+    fc = pkg.FccPlan(dims, np.array([0.31, -0.17, 0.23]), lat, delta)
+    v24 = torch.randn(24, N, dtype=torch.complex128, device=dev, generator=g)
+    rec("fcc_T", _time_op(torch, lambda: fc.T(v24), reps), 24, 32.0 * N, "vectors")
+    rec("fcc_TH", _time_op(torch, lambda: fc.TH(v24), reps), 24, 32.0 * N, "vectors")
+    x32 = torch.randn(32 * N, dtype=torch.complex128, device=dev, generator=g)
+    c24 = torch.randn(24 * N, dtype=torch.complex128, device=dev, generator=g)
+    phi = c24.abs() + 1.0 + 0j
+    rec("fcc_nfgep_A", _time_op(torch, lambda: fc.mtx_prod_vec_A_NFGEP(x32, c24, c24.conj(), phi), max(3, reps // 4)),
+        1, (64.0 * 16 + 3 * 16.0 * 24) * N, "matvecs")
+    sets = [r.random(N) < 0.3 for _ in range(4)]
+    ten = [r.standard_normal((3, 3)) + 1j * r.standard_normal((3, 3)) for _ in range(8)]
+    fc.mtx_Eps_Mu_Zeta_Xi(sets, sets, ten[:4], ten[4:])
+    x48 = torch.randn(48 * N, dtype=torch.complex128, device=dev, generator=g)
+    rec("fcc_gep_B", _time_op(torch, lambda: fc.GEP_B(x48), reps), 1, (2 * 48 * 16.0 + 2) * N, "matvecs")
+    fc.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# other workloads (--config 3/4/5); each prints its own JSON line
+# ---------------------------------------------------------------------------
+
+def run_config3(pkg, torch, args):
+    """64 k-points over three FCC sphere crystals (r = 0.12a, eps = 13, gamma ~ U[0, 3.4] from
+    default_rng(3)) on grids n = 48, 60, 72; 12 eigenpairs per k.  Batched: one batched plan per
+    grid, the three Lanczos solves running concurrently on their own streams (host threads);
+    plain: one k-point at a time (a sample of k-points per grid, mean per k-point)."""
+    import threading
+
+    S = pkg.structures
+    A = S.FCC_LATTICE
+    rng = np.random.default_rng(3)
+    grids = [48, 60, 72]
+    counts = [21, 21, 22]
+    gammas = rng.uniform(0, 3.4, 3)
+    kpts = S.fcc_band_path(sum(counts))
+    jobs, k0 = [], 0
+    for n, c, gm in zip(grids, counts, gammas):
+        dims = (n, n, n)
+        th = np.stack([S.theta_of_k(k, A) for k in kpts[k0:k0 + c]])
+        jobs.append((dims, th, S.bi_isotropic_table(13.0, gm, A), S.sphere_ids(dims, 0.12, A)))
+        k0 += c
+    nev = 12
+    lz = dict(tol=1e-10, maxiter=10000, p=2 * nev, cg_tol=1e-12, cg_maxit=10000)
+    # plain sample: 2 k-points per grid
+    plain = []
+    for dims, th, tab, ids in jobs:
+        for j in (0, len(th) // 2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pj = pkg.Plan(dims, th[j], tab, ids)
+            pj.Lanczos(nev, **lz)
+            torch.cuda.synchronize()
+            plain.append((dims[0], j, time.perf_counter() - t0))
+            pj.close()
+            log(f"config3: plain n={dims[0]} k#{j}: {plain[-1][2]:.2f} s")
+    # batched, the three grids concurrently
+    streams = [torch.cuda.Stream() for _ in jobs]
+    res, errs = [None] * 3, []
+
+    def work(i):
+        try:
+            with torch.cuda.stream(streams[i]):
+                dims, th, tab, ids = jobs[i]
+                pl = pkg.Plan(dims, th, tab, ids)
+                pl.autotune()
+                t0 = time.perf_counter()
+                ev, ew, it = pl.Lanczos(nev, **lz)
+                streams[i].synchronize()
+                res[i] = (time.perf_counter() - t0, it.tolist())
+                pl.close()
+        except Exception as e:  # reported below
+            errs.append(repr(e))
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    th_ = [threading.Thread(target=work, args=(i,)) for i in range(3)]
+    for t in th_:
+        t.start()
+    for t in th_:
+        t.join()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if errs:
+        raise RuntimeError("; ".join(errs))
+    per_grid_plain = {n: float(np.mean([t for (m, _, t) in plain if m == n])) for n in grids}
+    plain_total = sum(per_grid_plain[n] * c for n, c in zip(grids, counts))
+    out = {"metric": "s per k-point, 64 k over FCC spheres on n=48/60/72 (batched vs plain)",
+           "value": wall / 64, "unit": "s/k-point", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+           "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": "FCC spheres r=0.12a eps=13, gamma~U[0,3.4] (default_rng(3)), 12 eigenpairs, "
+                                  "n=48/60/72 with 21/21/22 k-points on X-U-L-G-X-W-K",
+                      "gammas": gammas.tolist(), "counts": counts},
+           "batched_wall_s": wall, "batched_solve_s_per_grid": [r[0] for r in res],
+           "lanczos_steps": [r[1] for r in res],
+           "plain_s_per_kpoint_per_grid": per_grid_plain, "plain_estimated_total_s": plain_total,
+           "s_per_kpoint_plain": plain_total / 64, "batched_over_plain": plain_total / wall,
+           "plain_sample": [{"n": m, "k_index": j, "s": t} for (m, j, t) in plain]}
+    print(json.dumps(out), flush=True)
+
+
+def run_config4(pkg, torch, args):
+    """JD-SIRA with locking near the Pasteur limit: FCC spheres (r = 0.12a, gamma = 3.46427)
+    and regular tetrahedra (edge 0.3a, gamma = 0.5 and 1.009), eps = 13, at n; ``nev``
+    eigenpairs nearest the target 0.3 with shift-invert MINRES inner solves."""
+    S = pkg.structures
+    A = S.FCC_LATTICE
+    n = args.n
+    dims = (n, n, n)
+    k = S.fcc_band_path(8)[1]
+    theta = S.theta_of_k(k, A)
+    cases = [("sphere", 3.46427, S.sphere_ids(dims, 0.12, A)), ("tetra", 0.5, S.tetra_ids(dims, 0.3, A)),
+             ("tetra", 1.009, S.tetra_ids(dims, 0.3, A))]
+    runs = []
+    for name, gm, ids in cases:
+        pl = pkg.Plan(dims, theta, S.bi_isotropic_table(13.0, gm, A), ids)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = pkg.jdsira.GEP_AB_Herm_JDSIRA_Driver4(pl, args.nev, stop_tolerance=1e-8, target=args.target,
+                                                   target_type="CL", sigma_inner=args.target, seed=0)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        EV = r.ev
+        res = []
+        for j in range(args.nev):
+            u = EV[j]
+            rr = pl.fncB(u) - complex(r.ew[j]) * pl.fncA(u)
+            res.append(float(rr.norm() / u.norm()))
+        runs.append({"structure": name, "gamma": gm, "seconds": dt, "omega": np.sort(r.ew.real).tolist(),
+                     "max_residual": max(res), "inner_iterations": r.total_inner,
+                     "outer_iterations": r.outer_iterations})
+        log(f"config4: {name} gamma={gm}: {dt:.1f} s, omega[:4]={np.sort(r.ew.real)[:4]}, max res {max(res):.2e}")
+        pl.close()
+    out = {"metric": "s per JD-SIRA solve near the Pasteur limit", "value": float(np.mean([r['seconds'] for r in runs])),
+           "unit": "s/solve", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+           "vs_baseline": None,
+           "config": {"workload": f"FCC n={n}, {args.nev} eigenpairs nearest {args.target}, CL target, "
+                                  f"shift-invert MINRES (tol 1e-2), stop tolerance 1e-8", "k": k.tolist()},
+           "runs": runs}
+    print(json.dumps(out), flush=True)
+
+
+def run_config5(pkg, torch, args):
+    """Adjoint eigenvalue gradients: SC woodpile at n (48), eps_v ~ U[1, 13] (default_rng(5)) on
+    the rod vertices, vacuum outside, gamma = 0.2 in the rods; the 8 lowest omega at one k and
+    their gradients by every vertex eps and by gamma, checked (a) per vertex on 64 random
+    vertices (default_rng(55)) against central differences (step 1e-6) of the Rayleigh
+    quotient through the device material apply, (b) along a random direction against central
+    differences (step 1e-4) of re-solved eigenvalues."""
+    S = pkg.structures
+    n = args.n
+    dims = (n, n, n)
+    N = n ** 3
+    ids = S.woodpile_sc_ids(dims)
+    eps = np.where(ids == 1, np.random.default_rng(5).uniform(1, 13, N), 1.0)
+    gam = np.where(ids == 1, 0.2, 0.0)
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    pl = pkg.Plan(dims, theta, S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), ids)
+    pl.set_vertex_material(eps, gam)
+    nev = 8
+    lz = dict(tol=1e-12, maxiter=2000, p=2 * nev + 4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bg = pkg.adjoint.BandGradient(pl)
+    omega = bg.forward(nev, **lz)[0]
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    d_eps, d_gam = pkg.adjoint.eigen_gradients(pl, bg.ev, bg.omega)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    d_eps, d_gam = d_eps.cpu().numpy()[0], d_gam.cpu().numpy()[0]
+    rod = ids == 1
+    g_gamma = d_gam[:, rod].sum(axis=1)
+    gap_eps = d_eps[4] - d_eps[3]
+    # (a) per-vertex central differences of x^H fncB x through the material apply
+    x = bg.ev[0]
+    h = 1e-6
+    verts = np.random.default_rng(55).choice(np.nonzero(rod)[0], 64, replace=False)
+    den = np.array([float(torch.vdot(x[j], pl.fncA(x[j])).real) for j in range(nev)])
+    fields = [torch.cat([pl.Pr(x[j][:2 * N]), pl.Qr(x[j][2 * N:])]) for j in range(nev)]
+    worst = 0.0
+    for v in verts:
+        ys = []
+        for sgn in (1, -1):
+            e2 = eps.copy()
+            e2[v] += sgn * h
+            pl.set_vertex_material(e2, gam)
+            ys.append([pl.material_apply(f) for f in fields])
+        # the difference vector is exactly zero away from vertex v, so no cancellation in the sum
+        fd = np.array([float(torch.vdot(f, yp - ym).real) for f, yp, ym in zip(fields, ys[0], ys[1])]) / (2 * h) / den
+        err = np.abs(fd - d_eps[:, v]) / np.maximum(np.abs(d_eps[:, v]), 1e-300)
+        worst = max(worst, float(err.max()))
+    pl.set_vertex_material(eps, gam)
+    # (b) random direction through re-solved eigenvalues
+    dirn = np.random.default_rng(56).standard_normal(N) * rod
+    tstep = 1e-4  # re-solved eigenvalues carry ~1e-14 relative error: a 1e-6 step would measure that
+    ws = []
+    for sgn in (1, -1):
+        pl.set_vertex_material(eps + sgn * tstep * dirn, gam)
+        ws.append(pkg.adjoint.BandGradient(pl).forward(nev, **lz)[0])
+    fd_dir = (ws[0] - ws[1]) / (2 * tstep)
+    pred_dir = d_eps @ dirn
+    dir_err = float(np.max(np.abs(fd_dir - pred_dir) / np.abs(pred_dir)))
+    pl.close()
+    out = {"metric": "adjoint eigenvalue gradients (lowest 8 omega, band 4/5 gap) by every vertex eps and gamma",
+           "value": t_bwd, "unit": "s per backward (8 eigenvectors)", "higher_is_better": False, "n_gpus": 1,
+           "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": f"SC woodpile n={n}, rod eps~U[1,13] (default_rng(5)), gamma=0.2, k=pi(0.5,0.3,0.1)"},
+           "forward_s": t_fwd, "backward_s": t_bwd, "omega": omega.tolist(),
+           "domega_dgamma": g_gamma.tolist(), "dgap45_deps_norm": float(np.linalg.norm(gap_eps)),
+           "fd_vertex_check": {"vertices": 64, "step": h, "max_rel_err": worst},
+           "fd_direction_check": {"step": tstep, "max_rel_err": dir_err, "fd": fd_dir.tolist(),
+                                  "adjoint": pred_dir.tolist()}}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -82,6 +354,11 @@ def main():
     ap.add_argument("--nev", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ops", action="store_true", help="skip the per-operator rates")
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3, 4, 5],
+                    help="2: the 40-k woodpile band path (default); 3: multi-grid FCC spheres; "
+                         "4: JD-SIRA near the Pasteur limit; 5: adjoint gradients")
+    ap.add_argument("--target", type=float, default=0.3, help="JD-SIRA target (config 4)")
     ap.add_argument("--chunk", type=int, default=-1,
                     help="k-points per fncB chunk (-1: autotune on the plan, 0: whole batch)")
     args = ap.parse_args()
@@ -95,6 +372,14 @@ def main():
     import __graft_entry__ as ge
 
     pkg = ge.load_package()
+    if args.config != 2:
+        torch.cuda.set_device(0)
+        if args.config == 5 and args.n == 60:
+            args.n = 48
+        if args.config == 4 and args.nev == 10:
+            args.nev = 20
+        {3: run_config3, 4: run_config4, 5: run_config5}[args.config](pkg, torch, args)
+        return
     S = pkg.structures
     n, nk = args.n, args.nk
     dims = (n, n, n)
@@ -210,6 +495,13 @@ def main():
         "stage_ms": stage_avg,
         "roofline": roofline,
     }
+
+    if not args.no_ops:
+        log("bench: per-operator rates")
+        out["ops"] = op_rates(pkg, torch, n, nk)
+        for k_, v_ in out["ops"].items():
+            log(f"bench:   {k_:28s} {v_['per_s']:10.0f} {v_['unit']:18s} {v_['algo_GBs']:7.0f} GB/s "
+                f"({100 * v_['hbm_frac']:.0f}% of HBM peak)")
 
     # ---- s per k-point: full Lanczos solves (table setup + eigensolve, as the reference's time_eig window) ----
     if args.solve != "none":

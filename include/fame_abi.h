@@ -226,6 +226,10 @@ int fame_fcc_destroy(fame_fcc* plan);
 /* stream: a hipStream_t, NULL = the default stream */
 int fame_fcc_set_stream(fame_fcc* plan, void* stream);
 int fame_fcc_sync(fame_fcc* plan);
+/* Kernel selection (no reference counterpart): n1 = n2 = n3 with a specialised FFT length
+ * uses the compile-time four-step kernels by default; generic != 0 forces the runtime-radix
+ * kernels that cover every size (used to cross-check the two). */
+int fame_fcc_set_generic(fame_fcc* plan, int generic);
 
 /* v = p_mtx_T_prod_vec_FCC(v, FFT_parameter) on nvec vectors [nvec][n]
  * (light_locked_Huang/p_mtx_T_prod_vec_FCC.m:1-37); in == out allowed. */

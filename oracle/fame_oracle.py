@@ -41,9 +41,14 @@ Pinning status (what checks each function; see ``tests/test_oracle_pins.py``):
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import scipy.fft
 import scipy.sparse as sp
+
+# host threads for the FFTs: the CPU share given by OMP_NUM_THREADS, at most the machine's cores
+FFT_WORKERS = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
 
 # --------------------------------------------------------------------------
 # 1D quasi-periodic SVD pieces (subroutines/@DistOp)
@@ -181,11 +186,11 @@ def MtxVecMul_FFT(vec_in, dims, istrans, S, D=None):
     D = np.ones(3 * N) if D is None else D
     if istrans == "notranspose":
         x = D * vec_in
-        out = [scipy.fft.ifftn(_cube(x[c * N:(c + 1) * N], dims), workers=-1).ravel() * norm_c for c in range(3)]
+        out = [scipy.fft.ifftn(_cube(x[c * N:(c + 1) * N], dims), workers=FFT_WORKERS).ravel() * norm_c for c in range(3)]
         return S * np.concatenate(out)
     if istrans == "transpose":
         x = np.conj(S) * vec_in
-        out = [scipy.fft.fftn(_cube(x[c * N:(c + 1) * N], dims), workers=-1).ravel() / norm_c for c in range(3)]
+        out = [scipy.fft.fftn(_cube(x[c * N:(c + 1) * N], dims), workers=FFT_WORKERS).ravel() / norm_c for c in range(3)]
         return np.conj(D) * np.concatenate(out)
     raise ValueError(istrans)
 

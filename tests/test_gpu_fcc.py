@@ -42,7 +42,8 @@ def make(pkg, dims, k):
 
 
 K = np.array([0.31, -0.17, 0.23])
-DIMS = [(2, 3, 3), (4, 6, 6), (6, 3, 4), (12, 12, 12), (10, 9, 8)]
+DIMS = [(2, 3, 3), (4, 6, 6), (6, 3, 4), (12, 12, 12), (10, 9, 8), (24, 24, 24)]
+FAST = [8, 12, 16, 20, 24, 32, 40, 48, 60, 64, 72, 80, 90, 96]
 
 
 @pytest.mark.parametrize("dims", DIMS)
@@ -137,4 +138,19 @@ def test_material_operators(pkg, torch_mod):
         assert rel(plan.param_apply(name, t.as_tensor(x)).cpu().numpy(), m[name] @ x) < RTOL, name
     x = crand(48 * n)
     assert rel(plan.GEP_B(t.as_tensor(x)).cpu().numpy(), fc.gep_B(m) @ x) < RTOL
+    plan.close()
+
+
+@pytest.mark.parametrize("n", FAST)
+def test_specialised_T_equals_generic(pkg, torch_mod, n):
+    """The compile-time four-step kernels (n1 = n2 = n3 with a specialised length) give the
+    runtime-radix kernels' T and T^H to rounding."""
+    say(f"FCC specialised == generic n={n}")
+    plan, _ = make(pkg, (n, n, n), K)
+    x = torch_mod.as_tensor(crand(2, plan.n), device="cuda")
+    yf, zf = plan.T(x), plan.TH(x)
+    plan.set_generic(True)
+    yg, zg = plan.T(x), plan.TH(x)
+    assert float((yf - yg).norm() / yg.norm()) < 1e-13
+    assert float((zf - zg).norm() / zg.norm()) < 1e-13
     plan.close()
