@@ -382,6 +382,52 @@ class Plan:
                                                ctypes.c_void_p(out.data_ptr())))
         return out
 
+    def _sigma_dev(self):
+        if getattr(self, "_sig", None) is None:
+            self._sig = self.torch.as_tensor(self.Sigma_r(), device=self.device).to(self.torch.complex128)
+        return self._sig
+
+    def _rows(self, y):
+        return y.reshape(self.nb, -1)
+
+    def curl(self, e):
+        """C e for fields (nb, 3N), through the exact factorisation C = Pr diag(Sigma_r) Qr^H
+        (Funhand_Generate_Lanzcos.m:218; C is DistOp.C of EvaluteError_Chiral_no_inv.m:10)."""
+        return self._rows(self.Pr(self._sigma_dev() * self._rows(self.Qrs(e))))
+
+    def curl_H(self, h):
+        """C^H h = Qr diag(Sigma_r) Pr^H h (Funhand_Generate_Lanzcos.m:219)."""
+        return self._rows(self.Qr(self._sigma_dev() * self._rows(self.Prs(h))))
+
+    def EvaluteError_Chiral_no_inv(self, ev, omega, e_h):
+        """(res_NFSEP, res_ORGSEP), each (nb, nvec) on the host (fun_no_inv/EvaluteError_Chiral_no_inv.m:18-36):
+        res_NFSEP = ||fncB(x) - w fncA(x)||,
+        res_ORGSEP = ||[N a + J b; J^H a + M b] - i w [e; h]|| with a = C^H h, b = -C e.
+        ``ev`` (nb, nvec, 4N), ``omega`` (nb, nvec), ``e_h`` (nb, nvec, 6N) = [e; h] from
+        Eigen_Restoration_Chiral_no_inv."""
+        t = self.torch
+        N3 = 3 * self.N
+        ev = ev.to(device=self.device, dtype=t.complex128).reshape(self.nb, -1, 4 * self.N)
+        e_h = e_h.to(device=self.device, dtype=t.complex128).reshape(self.nb, -1, 6 * self.N)
+        om = np.asarray(omega, float).reshape(self.nb, -1)
+        nvec = ev.shape[1]
+        r_nf = np.zeros((self.nb, nvec))
+        r_org = np.zeros((self.nb, nvec))
+        for j in range(nvec):
+            w = t.as_tensor(om[:, j], device=self.device)[:, None]
+            x = ev[:, j].contiguous()
+            r = self._rows(self.fncB(x)) - w * self._rows(self.fncA(x))
+            r_nf[:, j] = t.linalg.vector_norm(r, dim=1).cpu().numpy()
+            e = e_h[:, j, :N3].contiguous()
+            h = e_h[:, j, N3:].contiguous()
+            a = self.curl_H(h)
+            b = -self.curl(e)
+            y = self._rows(self.material_apply(t.cat([b, a], dim=1).contiguous()))  # [M b + J^H a; J b + N a]
+            d1 = y[:, N3:] - 1j * w * e
+            d2 = y[:, :N3] - 1j * w * h
+            r_org[:, j] = t.sqrt(t.linalg.vector_norm(d1, dim=1) ** 2 + t.linalg.vector_norm(d2, dim=1) ** 2).cpu().numpy()
+        return r_nf, r_org
+
     def set_vertex_material(self, eps, gamma, mu=1.0, lattice=None):
         """Per-vertex bi-isotropic material (eps_v, gamma_v at every vertex, x fastest) in place of
         the id table; ``lattice`` has the lattice vectors as columns (default simple cubic)."""

@@ -233,7 +233,10 @@ class _State:
         self.flag_no_ew = 0
         self.dim_eigsp = 0
         self.total_inner_iter = 0
-        self.log = []
+        self.log = []      # (ew_order, EPIteNo, ritz value, residual) per outer step (Locking4.m:184-195)
+        self.locked = []   # (ew_order, EPIteNo, seconds, eigenvalue, residual) per lock (output.m)
+        self.inner = []    # total inner iterations per locking call (Driver4.m:65)
+        self.t0 = None
 
 
 def _projected_eig(M, L):
@@ -362,6 +365,8 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
     st.flag_no_ew += 1
     k = st.dim_eigsp
     u = ev_crnt / _norm(plan, ev_crnt)  # lock the converged pair (:425-450)
+    import time as _time
+    st.locked.append((ew_order, EPIteNo, _time.perf_counter() - st.t0, complex(ritz_ew[0]), rsdl))
     V[k] = u
     EV[k] = u
     k += 1
@@ -420,11 +425,15 @@ def GEP_AB_Herm_JDSIRA_Driver4(plan, ew_number, stop_tolerance=1.0e-8, no_restar
     no_computed_ew = 1
     total_outer = 0
     rng = np.random.default_rng(seed + 1)
+    import time as _time
+    st.t0 = _time.perf_counter()
     while no_computed_ew <= ew_number:
+        inner0 = st.total_inner_iter
         no_computed_ew, proj_dim, epit, stop_flag = _locking(plan, A, B, EV, no_computed_ew, init_vec, proj_dim, target,
                                                   target_type, rppd, no_restart, stop_tolerance, ew_number, CorEq,
                                                   ls, st, sigma_inner)
         total_outer += epit
+        st.inner.append(st.total_inner_iter - inner0)
         if stop_flag == 0:
             raise NotConverged(FAME_ERR_NOT_CONVERGED, f"JD-SIRA: no convergence within 6000 outer steps "
                                                        f"({no_computed_ew - 1} of {ew_number} eigenpairs)")
@@ -432,4 +441,7 @@ def GEP_AB_Herm_JDSIRA_Driver4(plan, ew_number, stop_tolerance=1.0e-8, no_restar
             v = t.as_tensor(rng.standard_normal(L_len) + 0j, device=plan.device)
             init_vec[0] = v / np.sqrt(float(np.linalg.norm(v.cpu().numpy())))
             proj_dim = 1
-    return JDResult(st.zflag_ew[:ew_number].copy(), EV, st.log, st.total_inner_iter, total_outer)
+    res = JDResult(st.zflag_ew[:ew_number].copy(), EV, st.log, st.total_inner_iter, total_outer)
+    res.locked = st.locked
+    res.inner_per_lock = st.inner
+    return res

@@ -1,0 +1,62 @@
+"""GPU: device residual evaluation (EvaluteError_Chiral_no_inv) against the oracle with the
+explicit sparse curl, and the band-structure driver (plain and batched) end to end."""
+import numpy as np
+import pytest
+import scipy.io
+
+pytestmark = pytest.mark.gpu
+rng = np.random.default_rng(91)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def test_residuals_match_oracle(pkg, fo, torch_mod):
+    print("EvaluteError_Chiral_no_inv vs oracle", flush=True)
+    dims = (6, 5, 4)
+    S = pkg.structures
+    thetas = np.array([[0.4, -1.3, 2.2], [1.5, 0.1, -0.7]])
+    table, ids = S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), S.woodpile_sc_ids(dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    N = plan.N
+    ev = crand(2, 3, 4 * N)          # not eigenvectors: O(1) residuals, compared to 1e-12
+    omega = rng.uniform(0.5, 2, (2, 3))
+    e_h = crand(2, 3, 6 * N)
+    rn, ro = plan.EvaluteError_Chiral_no_inv(torch_mod.as_tensor(ev), omega, torch_mod.as_tensor(e_h))
+    T = table[ids]
+    for b in range(2):
+        P = fo.Problem(dims, thetas[b], T[:, 0], T[:, 1], T[:, 2])
+        C = fo.curl_C(dims, P.delta, thetas[b])
+        e = e_h[b][:, :3 * N].T
+        h = e_h[b][:, 3 * N:].T
+        r_nf, r_org = fo.evaluate_error(ev[b].T, omega[b], e, h, C, P.fncA, P.fncB, P.mtx_B)
+        assert np.allclose(rn[b], r_nf, rtol=1e-12, atol=0)
+        assert np.allclose(ro[b], r_org, rtol=1e-12, atol=0)
+    plan.close()
+
+
+def test_band_structure_driver_plain_and_batched(pkg, torch_mod, tmp_path):
+    print("band-structure driver n=8, 3 k", flush=True)
+    dims = (8, 8, 8)
+    S = pkg.structures
+    ks = S.sc_band_path(part_num=2)[:3]
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in ks])
+    table, ids = S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), S.woodpile_sc_ids(dims)
+    path = tmp_path / "band.mat"
+    rb = pkg.driver.band_structure_lanczos(pkg, dims, thetas, table, ids, nev=4, batched=True, mat_path=str(path))
+    rp = pkg.driver.band_structure_lanczos(pkg, dims, thetas, table, ids, nev=4, batched=False)
+    assert np.allclose(rb["Freq_mtx"], rp["Freq_mtx"], rtol=1e-10)
+    assert rb["res_NFSEP"].max() < 1e-8 and rp["res_NFSEP"].max() < 1e-8
+    assert rb["res_ORGSEP"].max() < 1e-6
+    m = scipy.io.loadmat(str(path))
+    assert np.allclose(m["Freq_mtx"], rb["Freq_mtx"])
