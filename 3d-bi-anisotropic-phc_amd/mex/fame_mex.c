@@ -21,7 +21,15 @@
  *   y = fame_mex('Prs' | 'Qrs', h, x)             x: 3N x nb complex -> 2N x nb
  *   [y, iters, relres] = fame_mex('fncinvB', h, x, tol, maxit)
  *   [ev, ew, iters] = fame_mex('lanczos', h, k, tol, maxiter, p, v0)   v0: 4N x nb
+ *   [y, iters, relres] = fame_mex('minres', h, x, sigma, tol, maxit)     (fncB - sigma fncA) y = x
+ *   fame_mex('vertex_material', h, eps, gamma, mu, lattice)              eps, gamma: N x 1, lattice 3x3
+ *   g = fame_mex('material_gradient', h, ev)   ev: 4N x nvec (nb = 1) -> N x 3 x nvec
  *   fame_mex('plan_destroy', h)
+ *
+ *   f = fame_mex('fcc_create', [n1 n2 n3], vec_k, a1, a2, a3, delta)     SVDSingleCurl_FCC tables
+ *   y = fame_mex('fcc_T' | 'fcc_TH', f, x)                               x: n x nvec complex
+ *   y = fame_mex('fcc_nfgep', f, x, xi_d, zeta_d, Phi)                  x: 32n, coefficients 24n
+ *   fame_mex('fcc_destroy', f)
  */
 #include <stdint.h>
 #include <string.h>
@@ -77,9 +85,91 @@ static void apply(slot_t* s, op_t op, const mxArray* x, long long lin, long long
 static int psi(fame_plan* p, const void* a, void* b) { return fame_MtxVecMul_FFT(p, 0, a, b); }
 static int psis(fame_plan* p, const void* a, void* b) { return fame_MtxVecMul_FFT(p, 1, a, b); }
 
+/* FCC plans: handle + device staging sized on demand */
+typedef struct {
+  fame_fcc* plan;
+  long long n;
+  void* din;
+  void* dout;
+  size_t cap;  /* bytes of each staging buffer */
+} fslot_t;
+
+static fslot_t* fslot_of(const mxArray* a) {
+  if (!mxIsUint64(a) || mxGetNumberOfElements(a) != 1) mexErrMsgIdAndTxt("fame:handle", "bad FCC plan handle");
+  return (fslot_t*)(uintptr_t)(*(uint64_t*)mxGetData(a));
+}
+
+static void fstage(fslot_t* f, size_t bytes) {
+  if (bytes <= f->cap) return;
+  if (f->din) hipFree(f->din);
+  if (f->dout) hipFree(f->dout);
+  hip_check(hipMalloc(&f->din, bytes));
+  hip_check(hipMalloc(&f->dout, bytes));
+  f->cap = bytes;
+}
+
+static void* dev_copy(const mxArray* x, size_t count) {
+  void* d = NULL;
+  if (!mxIsComplex(x) || mxGetNumberOfElements(x) != count) mexErrMsgIdAndTxt("fame:arg", "complex array of the right length expected");
+  hip_check(hipMalloc(&d, count * 16));
+  hip_check(hipMemcpy(d, mxGetComplexDoubles(x), count * 16, hipMemcpyHostToDevice));
+  return d;
+}
+
+static int fcc_command(const char* cmd, int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
+  if (!strcmp(cmd, "fcc_create")) {
+    if (nrhs != 7) mexErrMsgIdAndTxt("fame:arg", "fcc_create(dims, vec_k, a1, a2, a3, delta)");
+    const double* d = mxGetDoubles(prhs[1]);
+    fslot_t* f = (fslot_t*)mxCalloc(1, sizeof(fslot_t));
+    mexMakeMemoryPersistent(f);
+    check(fame_fcc_create(&f->plan, (int)d[0], (int)d[1], (int)d[2], mxGetDoubles(prhs[2]), mxGetDoubles(prhs[3]),
+                          mxGetDoubles(prhs[4]), mxGetDoubles(prhs[5]), mxGetDoubles(prhs[6])));
+    f->n = (long long)d[0] * (long long)d[1] * (long long)d[2];
+    plhs[0] = mxCreateNumericMatrix(1, 1, mxUINT64_CLASS, mxREAL);
+    *(uint64_t*)mxGetData(plhs[0]) = (uint64_t)(uintptr_t)f;
+    return 1;
+  }
+  if (strncmp(cmd, "fcc_", 4)) return 0;
+  if (nrhs < 2) mexErrMsgIdAndTxt("fame:arg", "missing FCC plan handle");
+  fslot_t* f = fslot_of(prhs[1]);
+  if (!strcmp(cmd, "fcc_destroy")) {
+    fame_fcc_destroy(f->plan);
+    if (f->din) hipFree(f->din);
+    if (f->dout) hipFree(f->dout);
+    mxFree(f);
+  } else if (!strcmp(cmd, "fcc_T") || !strcmp(cmd, "fcc_TH")) {
+    const size_t cnt = mxGetNumberOfElements(prhs[2]);
+    if (!mxIsComplex(prhs[2]) || cnt % (size_t)f->n) mexErrMsgIdAndTxt("fame:arg", "x: n x nvec complex");
+    fstage(f, cnt * 16);
+    hip_check(hipMemcpy(f->din, mxGetComplexDoubles(prhs[2]), cnt * 16, hipMemcpyHostToDevice));
+    const int nvec = (int)(cnt / (size_t)f->n);
+    check(cmd[5] == 'H' ? fame_fcc_TH(f->plan, nvec, f->din, f->dout) : fame_fcc_T(f->plan, nvec, f->din, f->dout));
+    check(fame_fcc_sync(f->plan));
+    plhs[0] = mxCreateDoubleMatrix((mwSize)f->n, (mwSize)nvec, mxCOMPLEX);
+    hip_check(hipMemcpy(mxGetComplexDoubles(plhs[0]), f->dout, cnt * 16, hipMemcpyDeviceToHost));
+  } else if (!strcmp(cmd, "fcc_nfgep")) {
+    const size_t n32 = 32 * (size_t)f->n, n24 = 24 * (size_t)f->n;
+    fstage(f, n32 * 16);
+    void* c[3];
+    for (int i = 0; i < 3; ++i) c[i] = dev_copy(prhs[3 + i], n24);
+    if (!mxIsComplex(prhs[2]) || mxGetNumberOfElements(prhs[2]) != n32) mexErrMsgIdAndTxt("fame:arg", "x: 32n complex");
+    hip_check(hipMemcpy(f->din, mxGetComplexDoubles(prhs[2]), n32 * 16, hipMemcpyHostToDevice));
+    check(fame_fcc_nfgep_A(f->plan, f->din, f->dout, c[0], c[1], c[2]));
+    check(fame_fcc_sync(f->plan));
+    plhs[0] = mxCreateDoubleMatrix((mwSize)n32, 1, mxCOMPLEX);
+    hip_check(hipMemcpy(mxGetComplexDoubles(plhs[0]), f->dout, n32 * 16, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; ++i) hipFree(c[i]);
+  } else {
+    mexErrMsgIdAndTxt("fame:arg", "unknown command %s", cmd);
+  }
+  (void)nlhs;
+  return 1;
+}
+
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   char cmd[32];
   if (nrhs < 1 || mxGetString(prhs[0], cmd, sizeof cmd)) mexErrMsgIdAndTxt("fame:arg", "first argument: command");
+  if (fcc_command(cmd, nlhs, plhs, nrhs, prhs)) return;
   if (!strcmp(cmd, "plan_create")) {
     if (nrhs != 6) mexErrMsgIdAndTxt("fame:arg", "plan_create(dims, theta, delta, mat_table, mat_id)");
     const double* d = mxGetDoubles(prhs[1]);
@@ -179,6 +269,44 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     }
     mxFree(ew);
     mxFree(it);
+  } else if (!strcmp(cmd, "minres")) {
+    const double sigma = mxGetScalar(prhs[3]), tol = mxGetScalar(prhs[4]);
+    const int maxit = (int)mxGetScalar(prhs[5]);
+    int* it = (int*)mxMalloc(sizeof(int) * s->nb);
+    double* rr = (double*)mxMalloc(sizeof(double) * s->nb);
+    to_dev(s, prhs[2], 4 * N, s->din);
+    check(fame_minres(s->plan, s->din, s->dout, sigma, tol, maxit, it, rr));
+    plhs[0] = from_dev(s, s->dout, 4 * N);
+    if (nlhs > 1) {
+      plhs[1] = mxCreateDoubleMatrix(1, s->nb, mxREAL);
+      for (int b = 0; b < s->nb; ++b) mxGetDoubles(plhs[1])[b] = it[b];
+    }
+    if (nlhs > 2) {
+      plhs[2] = mxCreateDoubleMatrix(1, s->nb, mxREAL);
+      memcpy(mxGetDoubles(plhs[2]), rr, sizeof(double) * s->nb);
+    }
+    mxFree(it);
+    mxFree(rr);
+  } else if (!strcmp(cmd, "vertex_material")) {
+    /* the 3x3 lattice arrives column-major; the ABI takes it row-major */
+    const double* L = mxGetDoubles(prhs[5]);
+    double A[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) A[3 * i + j] = L[i + 3 * j];
+    check(fame_plan_set_vertex_material(s->plan, mxGetDoubles(prhs[2]), mxGetDoubles(prhs[3]), mxGetScalar(prhs[4]), A));
+  } else if (!strcmp(cmd, "material_gradient")) {
+    if (s->nb != 1) mexErrMsgIdAndTxt("fame:arg", "material_gradient: plans with one k-point");
+    const int nvec = (int)mxGetN(prhs[2]);
+    void* dev_ev = dev_copy(prhs[2], (size_t)nvec * 4 * N);
+    void* dev_g = NULL;
+    hip_check(hipMalloc(&dev_g, (size_t)nvec * 3 * N * sizeof(double)));
+    check(fame_material_gradient(s->plan, nvec, dev_ev, dev_g));
+    check(fame_sync(s->plan));
+    const mwSize dims[3] = {(mwSize)N, 3, (mwSize)nvec};
+    plhs[0] = mxCreateNumericArray(3, dims, mxDOUBLE_CLASS, mxREAL);
+    hip_check(hipMemcpy(mxGetDoubles(plhs[0]), dev_g, (size_t)nvec * 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    hipFree(dev_ev);
+    hipFree(dev_g);
   } else {
     mexErrMsgIdAndTxt("fame:arg", "unknown command %s", cmd);
   }
