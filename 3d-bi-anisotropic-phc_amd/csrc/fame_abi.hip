@@ -88,6 +88,7 @@ struct fame_plan {
   double2* d_part2 = nullptr; // fused-CG p^H q partials from the MIX_OUT pass
   bool cg_fused = true;       // CG vector updates fused into fncB (specialised z kernels only)
   int cg_seq = 0;             // > 0: fused CG runs chunks of this many items to convergence one after another
+  int mt[3] = {0, 0, 0};      // material stencil tile TX, TY and slab TZ (0: heuristic)
   // per-vertex bi-isotropic material (eps_v, gamma_v at every vertex; replaces the id table)
   bool pv = false;
   double pv_G[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -422,6 +423,11 @@ int material(fame_plan* P, const double2* in, long long ibs, double2* out, long 
   if (P->fast) {
     int TX, TY, TZ;
     material_tile(P->n1, P->n2, P->n3, P->cnb, TX, TY, TZ);
+    if (P->mt[0] > 0) {  // tile chosen by the autotuner or the caller
+      TX = P->mt[0];
+      TY = P->mt[1];
+      if (P->mt[2] > 0) TZ = P->mt[2];
+    }
     const int ntile = ((P->n1 + TX - 1) / TX) * ((P->n2 + TY - 1) / TY);
     const int nzc = (P->n3 + TZ - 1) / TZ;
     hipLaunchKernelGGL(k_material_z, dim3(ntile, nzc, P->cnb), dim3(kMzThreads), material_z_lds(TX, TY), P->stream,
@@ -943,6 +949,35 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
     if (ms < best) { best = ms; bestc = c; }
   }
   P->chunk = bestc;
+  // material stencil tile and z-slab: the heuristic's best few shapes, each with three slab lengths
+  if (P->fast && !P->pv) {
+    struct Cand { double score; int tx, ty; };
+    std::vector<Cand> tiles;
+    for (int tx = 4; tx <= std::min(P->n1, 30); ++tx)
+      for (int ty = 4; ty <= std::min(P->n2, 30); ++ty) {
+        if ((tx + 2) * (ty + 2) > kMzThreads) continue;
+        const double pad = (double)P->n1 * P->n2 /
+                           ((double)((P->n1 + tx - 1) / tx * tx) * ((P->n2 + ty - 1) / ty * ty));
+        tiles.push_back({pad * tx * ty / (double)((tx + 2) * (ty + 2)), tx, ty});
+      }
+    std::sort(tiles.begin(), tiles.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
+    if (tiles.size() > 6) tiles.resize(6);
+    int hx, hy, hz;
+    material_tile(P->n1, P->n2, P->n3, (bestc > 0 ? bestc : P->nb), hx, hy, hz);
+    int btx = 0, bty = 0, btz = 0;
+    for (const Cand& c : tiles)
+      for (int tz : {hz, std::max(1, hz / 2), std::min(P->n3, 2 * hz)}) {
+        P->mt[0] = c.tx;
+        P->mt[1] = c.ty;
+        P->mt[2] = tz;
+        double ms;
+        RET_IF(time_it(ms));
+        if (ms < best) { best = ms; btx = c.tx; bty = c.ty; btz = tz; }
+      }
+    P->mt[0] = btx;
+    P->mt[1] = bty;
+    P->mt[2] = btz;
+  }
   P->prof = prof;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -955,6 +990,28 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
 int fame_plan_set_generic(fame_plan* P, int generic) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   P->fast = (generic == 0);
+  return FAME_OK;
+}
+
+int fame_plan_set_material_tile(fame_plan* P, int tx, int ty, int tz) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  if (tx == 0 && ty == 0) {
+    P->mt[0] = P->mt[1] = P->mt[2] = 0;
+    return FAME_OK;
+  }
+  if (tx < 1 || ty < 1 || tz < 0 || tx > P->n1 || ty > P->n2 || tz > P->n3 || (tx + 2) * (ty + 2) > kMzThreads)
+    return set_err(FAME_ERR_ARG, "material tile: need (tx+2)(ty+2) <= 256, tx <= n1, ty <= n2, 0 <= tz <= n3");
+  P->mt[0] = tx;
+  P->mt[1] = ty;
+  P->mt[2] = tz;
+  return FAME_OK;
+}
+
+int fame_plan_get_material_tile(fame_plan* P, int* tx, int* ty, int* tz) {
+  if (!P || !tx || !ty || !tz) return set_err(FAME_ERR_ARG, "NULL argument");
+  *tx = P->mt[0];
+  *ty = P->mt[1];
+  *tz = P->mt[2];
   return FAME_OK;
 }
 

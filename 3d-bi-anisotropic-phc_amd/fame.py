@@ -85,6 +85,8 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
         "fame_plan_set_cg_fused": ([vp, i], i),
+        "fame_plan_set_material_tile": ([vp, i, i, i], i),
+        "fame_plan_get_material_tile": ([vp, pi_, pi_, pi_], i),
         "fame_plan_set_chunk": ([vp, i], i),
         "fame_plan_get_chunk": ([vp], i),
         "fame_plan_autotune": ([vp, pd], i),
@@ -127,6 +129,7 @@ def exported_symbols():
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv",
         "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused",
+        "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
         "fame_plan_set_vertex_material", "fame_material_gradient",
@@ -242,6 +245,15 @@ class Plan:
         _check(self.lib.fame_plan_set_generic(self.h, int(bool(generic))))
 
     STAGES = ("zpass_mix_in", "xypass_inv", "material", "xypass_fwd", "zpass_mix_out")
+
+    def set_material_tile(self, tx, ty, tz=0):
+        """Material-stencil tile (tx, ty columns, tz slices; 0, 0 = heuristic)."""
+        _check(self.lib.fame_plan_set_material_tile(self.h, int(tx), int(ty), int(tz)))
+
+    def material_tile(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.fame_plan_get_material_tile(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def set_cg_fused(self, fused=True, seq_chunk=0):
         """CG schedule for fncinvB and Lanczos (same iterates up to rounding): fused lock step

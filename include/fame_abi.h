@@ -99,6 +99,12 @@ int fame_plan_get_variant(fame_plan* plan, int* xy_variant, int* z_variant);
  * that cover every supported size (used to cross-check the two). */
 int fame_plan_set_generic(fame_plan* plan, int generic);
 
+/* Material-stencil tile (no reference counterpart): TX x TY columns per work-group
+ * ((TX+2)(TY+2) <= 256) marching TZ slices (0 = the host heuristic for TZ); tx = ty = 0
+ * restores the heuristic.  fame_plan_autotune sets it. */
+int fame_plan_set_material_tile(fame_plan* plan, int tx, int ty, int tz);
+int fame_plan_get_material_tile(fame_plan* plan, int* tx, int* ty, int* tz);
+
 /* CG schedule (no reference counterpart; same iterates up to rounding): fused = 1
  * (default) folds the CG vector updates and the p^H q reduction into the first and
  * last pass of fncB and advances all k-points in lock step, chunk by chunk;
