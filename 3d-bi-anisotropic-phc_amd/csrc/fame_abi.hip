@@ -89,6 +89,7 @@ struct fame_plan {
   bool cg_fused = true;       // CG vector updates fused into fncB (specialised z kernels only)
   int cg_seq = 0;             // > 0: fused CG runs chunks of this many items to convergence one after another
   int mt[3] = {0, 0, 0};      // material stencil tile TX, TY and slab TZ (0: heuristic)
+  bool cg_graph = false;      // fused CG iterations replayed from a captured HIP graph
   // per-vertex bi-isotropic material (eps_v, gamma_v at every vertex; replaces the id table)
   bool pv = false;
   double pv_G[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -641,7 +642,40 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
       if (hs[i].active != 0.0) done = false;
     return FAME_OK;
   };
-  if (P->cg_seq > 0) {
+  if (P->cg_graph && !P->prof && P->cg_seq == 0) {
+    // one iteration over all chunks captured once as a HIP graph on the plan's own stream and
+    // replayed on the plan's stream (the pointers do not change within the solve)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t saved = P->stream;
+    HIP_TRY(hipStreamBeginCapture(P->own_stream, hipStreamCaptureModeThreadLocal));
+    P->stream = P->own_stream;
+    const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
+    int rc = FAME_OK;
+    for (int b0 = 0; b0 < nb && rc == FAME_OK; b0 += c) rc = step(b0, std::min(c, nb - b0));
+    P->stream = saved;
+    hipError_t ec = hipStreamEndCapture(P->own_stream, &graph);
+    if (rc != FAME_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return set_err(FAME_ERR_HIP, std::string("CG graph capture: ") + hipGetErrorString(ec));
+    ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ec != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return set_err(FAME_ERR_HIP, std::string("CG graph instantiate: ") + hipGetErrorString(ec));
+    }
+    bool done = false;
+    for (int it = 0; it < maxit && !done && rc == FAME_OK;) {
+      ec = hipGraphLaunch(exec, P->stream);
+      if (ec != hipSuccess) rc = set_err(FAME_ERR_HIP, std::string("CG graph launch: ") + hipGetErrorString(ec));
+      ++it;
+      if (rc == FAME_OK && (it % 4 == 0 || it == maxit)) rc = finished(0, nb, done);
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    RET_IF(rc);
+  } else if (P->cg_seq > 0) {
     // chunk by chunk to convergence: a chunk's vectors and fncB intermediates stay in the
     // last-level cache for all its iterations
     const int c = std::min(P->cg_seq, nb);
@@ -990,6 +1024,12 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
 int fame_plan_set_generic(fame_plan* P, int generic) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   P->fast = (generic == 0);
+  return FAME_OK;
+}
+
+int fame_plan_set_cg_graph(fame_plan* P, int on) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  P->cg_graph = (on != 0);
   return FAME_OK;
 }
 

@@ -85,6 +85,7 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
         "fame_plan_set_cg_fused": ([vp, i], i),
+        "fame_plan_set_cg_graph": ([vp, i], i),
         "fame_plan_set_material_tile": ([vp, i, i, i], i),
         "fame_plan_get_material_tile": ([vp, pi_, pi_, pi_], i),
         "fame_plan_set_chunk": ([vp, i], i),
@@ -128,7 +129,7 @@ def exported_symbols():
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv",
-        "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused",
+        "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused", "fame_plan_set_cg_graph",
         "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
@@ -245,6 +246,10 @@ class Plan:
         _check(self.lib.fame_plan_set_generic(self.h, int(bool(generic))))
 
     STAGES = ("zpass_mix_in", "xypass_inv", "material", "xypass_fwd", "zpass_mix_out")
+
+    def set_cg_graph(self, on=True):
+        """Replay the fused CG iteration from a captured HIP graph, or launch one by one (default)."""
+        _check(self.lib.fame_plan_set_cg_graph(self.h, int(bool(on))))
 
     def set_material_tile(self, tx, ty, tz=0):
         """Material-stencil tile (tx, ty columns, tz slices; 0, 0 = heuristic)."""

@@ -155,3 +155,56 @@ def fcc_band_path(n_k, part_num=None):
         f = (tt - cum[i]) / seglen[i]
         out.append(pts[i] + f * (pts[i + 1] - pts[i]))
     return np.array(out)
+
+
+# ---------------------------------------------------------------------------
+# FCC skewed-grid index sets (light_locked_Huang/Construct_idx_all_FCC.m)
+# ---------------------------------------------------------------------------
+
+def _fcc_sites(edge_len):
+    """S1..S5 of Construct_idx_all_FCC.m:150-155 (origin, a1, a2, a3, and the interior site)."""
+    r3, r6 = np.sqrt(3.0), np.sqrt(6.0)
+    return edge_len * np.array([[0, 0, 0], [1, 0, 0], [0.5, r3 / 2, 0], [0.5, r3 / 6, r6 / 3],
+                                [0.5, r3 / 6, r6 / 12]], float)
+
+
+def _fcc_locate(xyz, shape, rd_sphere, rd_cylinder, edge_len):
+    S = _fcc_sites(edge_len)
+    inside = np.zeros(xyz.shape[0], bool)
+    if shape == "cylinder":  # locate_cyld: rods S5-S_i (radius rd_cylinder, between the two sites)
+        for i in range(4):
+            d = S[i] - S[4]
+            L2 = d @ d
+            u = d / np.sqrt(L2)
+            proj = (xyz - S[4]) @ np.outer(u, u).T + S[4]
+            D1 = np.sum((xyz - proj) ** 2, axis=1)
+            D2 = np.sum((proj - S[i]) ** 2, axis=1)
+            D3 = np.sum((proj - S[4]) ** 2, axis=1)
+            inside |= (D1 <= rd_cylinder ** 2) & (D2 <= L2) & (D3 <= L2)
+    elif shape == "ellipse":  # locate_ellipse: prolate spheroids with foci S_i, S5
+        d5 = np.linalg.norm(xyz - S[4], axis=1)
+        for i in range(4):
+            s = np.linalg.norm(xyz - S[i], axis=1) + d5
+            inside |= s <= 2 * np.sqrt(rd_cylinder ** 2 + (np.linalg.norm(S[i] - S[4]) / 2) ** 2)
+    else:
+        raise ValueError("shape must be 'cylinder' or 'ellipse'")
+    for i in range(5):  # spheres at the five sites
+        inside |= np.sum((xyz - S[i]) ** 2, axis=1) <= rd_sphere ** 2
+    return inside
+
+
+def fcc_index_sets(dims, edge_len, rd_sphere, rd_cylinder, delta, shape="cylinder"):
+    """Inside masks of Construct_idx_all_FCC.m: grid points of each Yee location (edges X/Y/Z,
+    cube centres, faces X/Y/Z, vertices), half-offset by 0.5 mesh along the named axes, x fastest.
+    Returns (edge_sets, face_sets) = ([X, Y, Z, Cubic], [X, Y, Z, vertex]) boolean masks, the
+    inputs of mtx_Eps_Mu_Zeta_Xi (edge: edge X/Y/Z + Cubic; face: face X/Y/Z + vertex)."""
+    n1, n2, n3 = dims
+
+    def pts(ox, oy, oz):
+        i3, i2, i1 = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+        return np.stack([(i1.ravel() + ox) * delta[0], (i2.ravel() + oy) * delta[1], (i3.ravel() + oz) * delta[2]], 1)
+
+    loc = lambda o: _fcc_locate(pts(*o), shape, rd_sphere, rd_cylinder, edge_len)  # noqa: E731
+    edge = [loc((0.5, 0, 0)), loc((0, 0.5, 0)), loc((0, 0, 0.5)), loc((0.5, 0.5, 0.5))]
+    face = [loc((0, 0.5, 0.5)), loc((0.5, 0, 0.5)), loc((0.5, 0.5, 0)), loc((0, 0, 0))]
+    return edge, face

@@ -114,6 +114,11 @@ int fame_plan_get_material_tile(fame_plan* plan, int* tx, int* ty, int* tz);
  * and fall back to the textbook loop otherwise. */
 int fame_plan_set_cg_fused(fame_plan* plan, int fused);
 
+/* on != 0: the fused CG captures one iteration (all k-point chunks) as a HIP graph and
+ * replays it between convergence polls; 0 (default) launches the kernels one by one.
+ * Same kernels, same results (no reference counterpart). */
+int fame_plan_set_cg_graph(fame_plan* plan, int on);
+
 /* Measurement aid (no reference counterpart): with enable != 0 every fame_fncB
  * records HIP events on the plan's stream between its five kernels
  * (z-pass+Pi mix, xy-pass inverse, material stencil, xy-pass forward,

@@ -78,3 +78,34 @@ def test_sphere_and_tetra_fill(pkg):
     t = S.tetra_ids((48, 48, 48), 0.3, A)
     tv = 0.3 ** 3 / (6 * np.sqrt(2))
     assert abs(t.mean() - tv / vol) < 0.3 * tv / vol  # closed faces: boundary vertices count at n=48
+
+
+def test_fcc_index_sets(pkg):
+    """Construct_idx_all_FCC restatement: rods and spheres checked against an independent
+    parametric point-to-segment distance, Yee offsets and the site spheres."""
+    S = pkg.structures
+    dims = (12, 12, 12)
+    a = 1.0
+    delta = np.array([a / 12, a * np.sqrt(3) / 2 / 12, a * np.sqrt(6) / 3 / 12])
+    edge, face = S.fcc_index_sets(dims, a, 0.1, 0.06, delta, "cylinder")
+    assert len(edge) == 4 and len(face) == 4 and all(m.shape == (12 ** 3,) for m in edge + face)
+    assert face[3][0]  # the vertex at the origin sits in the sphere around S1
+    sites = S._fcc_sites(a)
+
+    def brute(o):
+        i3, i2, i1 = np.meshgrid(np.arange(12), np.arange(12), np.arange(12), indexing="ij")
+        P = np.stack([(i1.ravel() + o[0]) * delta[0], (i2.ravel() + o[1]) * delta[1], (i3.ravel() + o[2]) * delta[2]], 1)
+        ins = np.zeros(len(P), bool)
+        for i in range(4):
+            d = sites[i] - sites[4]
+            t = ((P - sites[4]) @ d) / (d @ d)
+            foot = sites[4] + np.outer(t, d)
+            ins |= (np.linalg.norm(P - foot, axis=1) <= 0.06 + 1e-12) & (t >= -1e-12) & (t <= 1 + 1e-12)
+        for s in sites:
+            ins |= np.linalg.norm(P - s, axis=1) <= 0.1 + 1e-12
+        return ins
+    for m, o in zip(edge + face, [(0.5, 0, 0), (0, 0.5, 0), (0, 0, 0.5), (0.5, 0.5, 0.5),
+                                  (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0), (0, 0, 0)]):
+        assert np.array_equal(m, brute(o))
+    e2, _ = S.fcc_index_sets(dims, a, 0.1, 0.06, delta, "ellipse")
+    assert e2[0].sum() >= edge[0].sum() * 0.5 and e2[0].any()

@@ -215,6 +215,31 @@ def test_fused_cg_matches_textbook_cg(pkg, torch_mod, n, chunk):
     plan.close()
 
 
+def test_cg_graph_replays_match_eager(pkg, torch_mod):
+    """The fused CG replayed from a captured HIP graph gives the eagerly launched result on
+    every call, with other device work and synchronisations between the calls."""
+    say("CG graph replay == eager")
+    dims = (12, 12, 12)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE)
+                       for k in ([0.5, 0.3, 0.1], [0.2, 0.0, 0.9], [0.7, 0.7, 0.1])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    plan.set_chunk(2)
+    for rep in range(3):
+        x = torch_mod.as_tensor(crand(3, 4 * plan.N)).to("cuda")
+        plan.set_cg_graph(True)
+        yg, itg, rg = plan.fncinvB(x, tol=1e-12, maxit=3000)
+        z = plan.fncB(x)            # eager work between the replays
+        torch_mod.cuda.synchronize()
+        plan.set_cg_graph(False)
+        ye, ite, re_ = plan.fncinvB(x, tol=1e-12, maxit=3000)
+        assert torch_mod.equal(yg, ye) and list(itg) == list(ite), rep
+        assert float((plan.fncB(ye) - x).norm() / x.norm()) < 2e-12
+        del z
+    plan.close()
+
+
 def pkg_fo():
     from oracle import fame_oracle
 
