@@ -265,15 +265,26 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
   } else {
     // phase A from HBM: item (c, t, pp)
     const double2* __restrict__ fin = in + b * in_bstride + (MODE == ZM_MIX_OUT ? (long long)(3 * slot) * N : 0);
-    __syncthreads();
-    for (int it = threadIdx.x; it < 3 * B * TP; it += NT) {
+    constexpr int IA = (3 * B * TP + NT - 1) / NT;
+    double2 ua[IA][A];
+#pragma unroll
+    for (int i = 0; i < IA; ++i) {  // all loads first, before the twiddle barrier
+      const int it = threadIdx.x + i * NT;
       const int pp = it % TP, r = it / TP;
       const int t = r % B, c = r / B;
       const int q = q0 + pp;
-      double2 u[A];
 #pragma unroll
       for (int j = 0; j < A; ++j)
-        u[j] = (q < n12) ? fin[c * N + q + (long long)n12 * (t + B * j)] : make_double2(0, 0);
+        ua[i][j] = (it < 3 * B * TP && q < n12) ? fin[c * N + q + (long long)n12 * (t + B * j)] : make_double2(0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < IA; ++i) {
+      const int it = threadIdx.x + i * NT;
+      if (it >= 3 * B * TP) break;
+      const int pp = it % TP, r = it / TP;
+      const int t = r % B, c = r / B;
+      double2* u = ua[i];
       rdft<A, DIR>(u, tw, B);
       L(c, t, pp) = u[0];
 #pragma unroll
