@@ -68,6 +68,30 @@ def cpu_baseline(n, seconds):
                       f"{cnt} calls in {dt:.1f} s; scipy.fft workers={fo.FFT_WORKERS}, CSR SpMV single-threaded"}
 
 
+def hbm_copy_rate(torch, nbytes=1 << 31, reps=10):
+    """Device-to-device copy of ``nbytes`` (2 GiB: far beyond the 256 MB Infinity Cache),
+    timed with HIP events; GB/s counting the read and the write."""
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    ms = _time_op(torch, lambda: dst.copy_(src), reps, warm=3)
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def read_traffic(path, items_per_launch):
+    """HBM bytes per launch from a profile summary, if it was measured at this launch size."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = float(v)
+    if vals.get("items_per_launch") != items_per_launch or "traffic_bytes_per_launch" not in vals:
+        return None
+    return vals["traffic_bytes_per_launch"]
+
+
 def _time_op(torch, fn, reps, warm=2):
     for _ in range(warm):
         fn()
@@ -373,6 +397,10 @@ def main():
     ap.add_argument("--jd-tol", type=float, default=1e-9,
                     help="JD-SIRA stop tolerance (config 4); pairs are accepted up to 10x it, so 1e-9 keeps "
                          "every residual below 1e-8")
+    ap.add_argument("--variant", type=int, nargs=2, metavar=("XY", "Z"),
+                    help="with --chunk: FFT launch shapes (xy 0/1, z 0/1/2) instead of the defaults")
+    ap.add_argument("--mtile", type=int, nargs=3, metavar=("TX", "TY", "TZ"),
+                    help="with --chunk: material-stencil tile instead of the heuristic")
     ap.add_argument("--chunk", type=int, default=-1,
                     help="k-points per fncB chunk (-1: autotune on the plan, 0: whole batch)")
     args = ap.parse_args()
@@ -422,11 +450,16 @@ def main():
 
     if args.chunk < 0:
         chunk, tune_ms = pb.autotune()
-        log(f"bench: autotuned chunk = {chunk}, xy/z launch variants = {pb.variant()} "
-            f"({tune_ms:.3f} ms per batched fncB)")
-    else:
+        log(f"bench: autotuned chunk = {chunk}, xy/z launch variants = {pb.variant()}, material tile = "
+            f"{pb.material_tile()} ({tune_ms:.3f} ms per batched fncB)")
+    else:  # a fixed configuration (no autotuning), e.g. the one autotune picked, for profiling
         chunk = args.chunk
         pb.set_chunk(chunk)
+        if args.variant:
+            pb.set_variant(*args.variant)
+        if args.mtile:
+            pb.set_material_tile(*args.mtile)
+        log(f"bench: fixed chunk = {chunk}, xy/z launch variants = {pb.variant()}, material tile = {pb.material_tile()}")
     x1, y1 = xb[0], yb[0]
 
     def run_plain():
@@ -472,17 +505,20 @@ def main():
     pb.profile(False)
     stage_avg = {k: v / calls for k, v in st.items()}
     dom = max(stage_avg, key=stage_avg.get)
-    dom_bytes = STAGE_BYTES_PER_POINT[dom] * N * nk
-    achieved = dom_bytes / (stage_avg[dom] * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_n{n}.txt")
-    if os.path.exists(tfile):
-        for line in open(tfile):
-            if line.startswith("traffic_bytes_per_launch:"):
-                traffic = float(line.split(":", 1)[1])
+    # fncB runs the batch in launches of `chunk` items (0 = all): per-launch figures
+    per_launch = chunk if 0 < chunk < nk else nk
+    launches = -(-nk // per_launch)
+    launch_bytes = STAGE_BYTES_PER_POINT[dom] * N * per_launch
+    launch_ms = stage_avg[dom] * per_launch / nk  # a full-size launch (the last one may be shorter)
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{dom}_n{n}.txt"), per_launch)
+    copy_gbs = hbm_copy_rate(torch)
+    log(f"bench: device copy {copy_gbs:.0f} GB/s (2 GiB, read + write)")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "algo_bytes_per_launch": dom_bytes}
+                "algo_bytes_per_launch": launch_bytes, "items_per_launch": per_launch,
+                "launches_per_step": launches, "avg_launch_us": launch_ms * 1e3,
+                "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs}
     log("bench: stage ms " + ", ".join(f"{k}={v:.4f}" for k, v in stage_avg.items()))
     fncb_gbs = FNCB_ALGO_BYTES_PER_POINT * N * value / 1e9
 
