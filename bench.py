@@ -152,7 +152,6 @@ def op_rates(pkg, torch, n, nk=40, reps=20):
     pl.close()
     del x4, x3, x2, ev
     # FCC skewed grid at the same n (one k, vectors batched)
-    from oracle import fcc_oracle  # noqa: F401  (lattice constants only)
     lat = (np.array([1.0, 0.0, 0.0]), np.array([0.5, np.sqrt(3) / 2, 0.0]), np.array([0.5, np.sqrt(3) / 6, np.sqrt(6) / 3]))
     delta = np.array([1 / n, np.sqrt(3) / 2 / n, np.sqrt(6) / 3 / n])
     fc = pkg.FccPlan(dims, np.array([0.31, -0.17, 0.23]), lat, delta)
