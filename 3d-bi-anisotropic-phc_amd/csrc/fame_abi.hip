@@ -86,6 +86,10 @@ struct fame_plan {
   double2* d_w2 = nullptr;    // [nb][6N]
   double2* d_part = nullptr;  // reduction partials
   double2* d_part2 = nullptr; // fused-CG p^H q partials from the MIX_OUT pass
+  double2* d_mr = nullptr;    // MINRES work vectors [6][nb][4N], allocated by the first fame_minres
+  void* d_mrst = nullptr;     // MINRES per-item state [nb]
+  double2* d_mgs = nullptr;   // fame_mgs coefficients when the caller passes none
+  int mgs_cap = 0;
   bool cg_fused = true;       // CG vector updates fused into fncB (specialised z kernels only)
   int cg_seq = 0;             // > 0: fused CG runs chunks of this many items to convergence one after another
   int mt[3] = {0, 0, 0};      // material stencil tile TX, TY and slab TZ (0: heuristic)
@@ -884,7 +888,7 @@ int fame_plan_destroy(fame_plan* P) {
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
-                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw};
+                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)

@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         "fame_minres": ([vp, vp, vp, d, d, i, pi_, pd], i),
         "fame_block_dot": ([vp, vp, i, vp, ctypes.c_longlong, vp], i),
         "fame_block_gemv": ([vp, vp, i, ctypes.c_longlong, vp, d, vp], i),
+        "fame_mgs": ([vp, vp, i, ctypes.c_longlong, vp, vp], i),
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
         "fame_plan_set_cg_fused": ([vp, i], i),
@@ -128,7 +129,7 @@ def exported_symbols():
         "fame_version", "fame_last_error", "fame_plan_create", "fame_plan_set_theta", "fame_plan_destroy",
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
-        "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv",
+        "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs",
         "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused", "fame_plan_set_cg_graph",
         "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
@@ -371,6 +372,18 @@ class Plan:
         _check(self.lib.fame_block_gemv(self.h, ctypes.c_void_p(V.data_ptr()), int(m), int(L),
                                         ctypes.c_void_p(y.data_ptr()), float(beta), ctypes.c_void_p(out.data_ptr())))
         return out
+
+    def mgs(self, V, t, coef=False):
+        """Modified Gram-Schmidt of t against the rows of V, in order, in place on the device
+        (GEP_AB_Herm_JDSIRA_Locking4.m:326-330); returns the coefficients V_j^H t (device) if
+        ``coef``."""
+        m, L = V.shape
+        if t.numel() != L or not t.is_contiguous() or not V.is_contiguous():
+            raise ValueError("mgs: V (m, len) and t (len,) must be contiguous device tensors")
+        c = self.torch.empty(m, dtype=self.torch.complex128, device=self.device) if coef else None
+        _check(self.lib.fame_mgs(self.h, ctypes.c_void_p(V.data_ptr()), int(m), int(L), ctypes.c_void_p(t.data_ptr()),
+                                 None if c is None else ctypes.c_void_p(c.data_ptr())))
+        return c
 
     def Lanczos(self, k, tol=1e-10, maxiter=10000, p=None, v0=None, cg_tol=1e-12, cg_maxit=10000):
         """Thick-restart Lanczos (Lanczos.m:1-156); returns (ev [nb,k,4N], ew [nb,k], iters [nb])."""

@@ -47,16 +47,20 @@ class JDResult:
 def gram_schmidt(plan, Y):
     """Modified Gram-Schmidt, dropping vectors of norm <= 1e-6 (Gram_Schmidt.m:1-25).
 
-    Y is a device tensor (n_Y, len) modified in place; returns the number kept."""
+    Y is a device tensor (n_Y, len) modified in place; returns the number kept.  In the
+    inner loop (Gram_Schmidt.m:13-16) each Y_j, j > k, meets only the new Y_nQ, so the
+    coefficients Y_nQ^H Y_j come from one block product and stay on the device."""
     n_Y = Y.shape[0]
     n_Q = 0
     for k in range(n_Y):
         temp = _norm(plan, Y[k])
         if abs(temp) > 1.0e-6:
             Y[n_Q] = Y[k] / temp
-            for j in range(k + 1, n_Y):
-                t = _dot(plan, Y[n_Q], Y[j])
-                _axpy(plan, -t, Y[n_Q], Y[j])
+            if k + 1 < n_Y:
+                c = -plan.block_dot(Y[k + 1:], Y[n_Q]).conj()     # -(Y_nQ^H Y_j)
+                q = Y[n_Q].reshape(1, -1)
+                for j in range(k + 1, n_Y):
+                    plan.block_gemv(q, c[j - k - 1:j - k], out=Y[j], beta=1.0)
             n_Q += 1
     return n_Q
 
@@ -69,18 +73,17 @@ def _norm(plan, x):
     return float(np.sqrt(abs(_dot(plan, x, x))))
 
 
-def _axpy(plan, a, x, y):
-    """y += a x on the device."""
-    t = plan.torch
-    plan.block_gemv(x.reshape(1, -1), t.tensor([a], dtype=t.complex128), out=y, beta=1.0)
-
-
 def choose_ritz_pair(iteno, ew, ew_order, target, VRR, vec_r_tol, choose_type, rppd, target_org, choose_type_org,
                      zflag_ew, flag_no_ew):
     """Select Ritz pairs (Choose_Ritz_Pair_Unsymm4.m:1-412); returns (ewno, ritz_ew, ritz_ev)."""
     ew = np.asarray(ew, complex)
     sort_ritz = ew.copy()
     sort_No = 5
+    # The reference tests closeness with a loop k = -1, 0, ..., K that stops at the first
+    # |a - b| <= 10^k * vec_r_tol; the thresholds grow with k, so the loop succeeds exactly
+    # when |a - b| <= 10^K * vec_r_tol (K = 1 for multiplicities, sort_No for the removal).
+    tol_mult = 10.0 * vec_r_tol
+    tol_remove = 10.0 ** sort_No * vec_r_tol
     if ew_order > 1:
         sort_ew = np.zeros(iteno, complex)
         sort_ew[:flag_no_ew] = zflag_ew[:flag_no_ew]
@@ -89,56 +92,29 @@ def choose_ritz_pair(iteno, ew, ew_order, target, VRR, vec_r_tol, choose_type, r
             if sort_ew[i].real > -1.0e13:
                 no_cn_ew = 1
                 for i1 in range(i + 1, flag_no_ew):
-                    if sort_ew[i1].real > -1.0e13:
-                        k, flag_tol = -1, 1
-                        while k <= 1 and flag_tol == 1:
-                            if abs(sort_ew[i1] - sort_ew[i]) <= 10.0 ** k * vec_r_tol:
-                                flag_tol = 0
-                            else:
-                                k += 1
-                        if flag_tol == 0:
-                            no_cn_ew += 1
-                            sort_ew[i1] = -1.0e14
+                    if sort_ew[i1].real > -1.0e13 and abs(sort_ew[i1] - sort_ew[i]) <= tol_mult:
+                        no_cn_ew += 1
+                        sort_ew[i1] = -1.0e14
             else:
                 no_cn_ew = 0
             mult_cn_ew[i] = no_cn_ew
         idx_ritz_w = np.zeros(iteno, int)  # 1-based links as in the reference (0 = first occurrence)
         mult_ritz_w = np.zeros(iteno, int)
+        close = np.abs(sort_ritz[:, None] - sort_ritz[None, :]) <= tol_mult
         for i in range(iteno):  # multiplicity of Ritz values (:97-121)
             if idx_ritz_w[i] == 0:
-                no_ritz_w = 1
-                for i1 in range(i + 1, iteno):
-                    if idx_ritz_w[i1] == 0:
-                        k, flag_tol = -1, 1
-                        while k <= 1 and flag_tol == 1:
-                            if abs(sort_ritz[i1] - sort_ritz[i]) <= 10.0 ** k * vec_r_tol:
-                                flag_tol = 0
-                            else:
-                                k += 1
-                        if flag_tol == 0:
-                            no_ritz_w += 1
-                            idx_ritz_w[i1] = i + 1
+                # the inner loop only links indices it has not passed yet, so its set of
+                # links is fixed by the state before it
+                js = i + 1 + np.nonzero(close[i, i + 1:] & (idx_ritz_w[i + 1:] == 0))[0]
+                idx_ritz_w[js] = i + 1
+                mult_ritz_w[i] = 1 + js.size
             else:
-                no_ritz_w = 0
-            mult_ritz_w[i] = no_ritz_w
+                mult_ritz_w[i] = 0
         for i in range(flag_no_ew):  # remove Ritz values equal to converged eigenvalues (:126-228)
             if sort_ew[i].real > -1.0e13:
-                i1, flag, ind_sort = 0, 1, -1
-                while i1 < iteno and flag == 1:
-                    if idx_ritz_w[i1] == 0:
-                        k, flag_tol = -1, 1
-                        while k <= sort_No and flag_tol == 1:
-                            if abs(sort_ritz[i1] - sort_ew[i]) <= 10.0 ** k * vec_r_tol:
-                                flag_tol = 0
-                                ind_sort = i1
-                            else:
-                                k += 1
-                        if flag_tol == 0:
-                            flag = 0
-                        else:
-                            i1 += 1
-                    else:
-                        i1 += 1
+                hits = np.nonzero((idx_ritz_w[:iteno] == 0) & (np.abs(sort_ritz[:iteno] - sort_ew[i]) <= tol_remove))[0]
+                flag = 0 if hits.size else 1
+                ind_sort = int(hits[0]) if hits.size else -1
                 if flag == 1:
                     continue  # 'Error in Choose_Ritz_Pair' (printed by the reference, no action)
                 if mult_cn_ew[i] == mult_ritz_w[ind_sort]:
@@ -326,9 +302,8 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
                     nrm = _norm(plan, tv)
                     if nrm < 1.0e-9:
                         tv = tv / nrm
-                    for j in range(iteno):  # orthogonalise against V (:326-330)
-                        c = _dot(plan, V[j], tv)
-                        _axpy(plan, -c, V[j], tv)
+                    tv = tv.contiguous()
+                    plan.mgs(V[:iteno], tv)  # orthogonalise against V, MGS in column order (:326-330)
                     V[iteno] = tv / _norm(plan, tv)
                     AV = A(V[iteno])  # update V^H A V and V^H B V (:335-341)
                     Mm[:iteno + 1, iteno] = plan.block_dot(V[:iteno + 1], AV).cpu().numpy()

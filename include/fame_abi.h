@@ -189,6 +189,14 @@ int fame_minres(fame_plan* plan, const void* in, void* out, double sigma, double
 int fame_block_dot(fame_plan* plan, const void* V, int m, const void* w, long long len, void* out);
 int fame_block_gemv(fame_plan* plan, const void* V, int m, long long len, const void* y, double beta, void* out);
 
+/* Modified Gram-Schmidt of t (len) against the m columns of V (column-major, len each), in
+ * column order, on the device: c_j = V_j^H t; t <- t - c_j V_j for j = 0..m-1
+ * (light_locked_Huang/GEP_AB_Herm_JDSIRA_Locking4.m:326-330, the orthogonalisation of the
+ * correction vector).  Each update is fused with the next inner product; the c_j stay on
+ * the device (written to coef, m complex, when coef is not NULL).  Asynchronous on the
+ * plan's stream, no host synchronisation. */
+int fame_mgs(fame_plan* plan, const void* V, int m, long long len, void* t, void* coef);
+
 /* Eigen_Restoration_Chiral_no_inv (fun_no_inv/Eigen_Restoration_Chiral_no_inv.m:3-25):
  * for nvec NFSEP vectors per item ([nb][nvec][4N]) returns [nb][nvec][6N] =
  * [e (3N); h (3N)] with e = -i (N Qr(v_b) + J Pr(v_a)), h = -i (J^H Qr(v_b) + M Pr(v_a)). */

@@ -67,6 +67,59 @@ def test_block_dot_gemv(pkg, torch_mod, m, length):
     plan.close()
 
 
+@pytest.mark.parametrize("m,length", [(1, 480), (9, 4 * 1000 + 4), (40, 4 * 13824)])
+def test_mgs_matches_sequential_and_orthogonalises(pkg, torch_mod, m, length):
+    """fame_mgs = the column-by-column MGS of Locking4.m:326-330 (non-orthonormal V: same
+    coefficients and result as the sequential loop), and against orthonormal columns the
+    result is orthogonal to every column (a plausible index or conjugation slip fails one)."""
+    say(f"mgs m={m} len={length}")
+    t = torch_mod
+    plan = pkg.Plan((4, 5, 6), THETAS[0], np.zeros((1, 3, 3, 3), complex) + np.eye(3), np.zeros(120, np.uint8))
+    V = crand(m, length) / np.sqrt(length)
+    x = crand(length)
+    ref, cref = x.copy(), np.zeros(m, complex)
+    for j in range(m):
+        cref[j] = np.vdot(V[j], ref)
+        ref = ref - cref[j] * V[j]
+    td = t.as_tensor(x, device="cuda")
+    c = plan.mgs(t.as_tensor(V, device="cuda"), td, coef=True)
+    assert rel(td.cpu().numpy(), ref) < 1e-12
+    assert rel(c.cpu().numpy(), cref) < 1e-12
+    Q = np.linalg.qr(V.T)[0].T.copy()                       # orthonormal rows
+    td = t.as_tensor(x, device="cuda")
+    plan.mgs(t.as_tensor(Q, device="cuda"), td)
+    r = td.cpu().numpy()
+    assert np.abs(Q.conj() @ r).max() <= 1e-13 * np.linalg.norm(x)
+    assert rel(r, x - Q.T @ (Q.conj() @ x)) < 1e-12
+    plan.close()
+
+
+def test_gram_schmidt_device_matches_mgs(pkg, torch_mod):
+    """jdsira.gram_schmidt (block coefficients per step) = Gram_Schmidt.m's MGS loop,
+    including the drop of a dependent vector (norm <= 1e-6 after orthogonalisation)."""
+    say("gram_schmidt device vs MGS loop")
+    t = torch_mod
+    plan = pkg.Plan((4, 5, 6), THETAS[0], np.zeros((1, 3, 3, 3), complex) + np.eye(3), np.zeros(120, np.uint8))
+    Y = crand(6, 4 * 1000)
+    Y[3] = 0.5 * Y[0] - 2j * Y[1]                            # dependent: dropped
+    ref = Y.copy()
+    nq = 0
+    for k in range(6):
+        nr = np.linalg.norm(ref[k])
+        if nr > 1e-6:
+            ref[nq] = ref[k] / nr
+            for j in range(k + 1, 6):
+                ref[j] = ref[j] - np.vdot(ref[nq], ref[j]) * ref[nq]
+            nq += 1
+    Yd = t.as_tensor(Y, device="cuda")
+    n_q = pkg.jdsira.gram_schmidt(plan, Yd)
+    assert n_q == nq == 5
+    got = Yd[:n_q].cpu().numpy()
+    assert rel(got, ref[:nq]) < 1e-12
+    assert np.abs(got.conj() @ got.T - np.eye(n_q)).max() < 1e-13
+    plan.close()
+
+
 @pytest.mark.parametrize("sigma", [0.0, 0.8])
 def test_minres_iterates_match_oracle(pkg, fo, torch_mod, sigma):
     say(f"minres iterates sigma={sigma}")

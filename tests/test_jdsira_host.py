@@ -42,6 +42,14 @@ class DensePlan:
         out.mul_(beta).add_(r)
         return out
 
+    def mgs(self, V, t, coef=False):
+        cs = []
+        for j in range(V.shape[0]):  # Locking4.m:326-330
+            c = self.torch.vdot(V[j], t)
+            t -= c * V[j]
+            cs.append(c)
+        return self.torch.stack(cs) if coef else None
+
     def minres(self, x, sigma=0.0, tol=1e-2, maxit=30000):
         from oracle import fame_oracle as fo
 
