@@ -64,7 +64,15 @@ def cpu_baseline(n, seconds):
         x /= np.linalg.norm(x)
         cnt += 1
     dt = time.perf_counter() - t0
-    return {"value": cnt / dt, "unit": "matvecs/s", "cores": fo.FFT_WORKERS, "kind": "port",
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    host = {"nproc": os.cpu_count(), "cpu_model": model,
+            "threads_env": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS")}}
+    return {"value": cnt / dt, "unit": "matvecs/s", "cores": fo.FFT_WORKERS, "kind": "port", "host": host,
             "sample": f"oracle fncB (CPU restatement of the reference algorithm, not MATLAB) at n={n}, one k-point, "
                       f"{cnt} calls in {dt:.1f} s; scipy.fft workers={fo.FFT_WORKERS}, CSR SpMV single-threaded"}
 
@@ -176,6 +184,80 @@ def op_rates(pkg, torch, n, nk=40, reps=20):
 # ---------------------------------------------------------------------------
 # other workloads (--config 3/4/5); each prints its own JSON line
 # ---------------------------------------------------------------------------
+
+def cpu_baseline_solve(n, theta, table, ids, nev, tol, p):
+    """CPU baseline of one k-point solve: the oracle's Lanczos (the reference's Lanczos.m with
+    pcg and fncB as scipy FFTs + CSR SpMV on the host cores), v0 = ones as the reference.
+    Returns (seconds, omega ascending, Lanczos steps, mean CG iterations per solve)."""
+    from oracle import fame_oracle as fo
+
+    dims = (n, n, n)
+    t0 = time.perf_counter()
+    T = table[ids]
+    P = fo.Problem(dims, theta, T[:, 0], T[:, 1], T[:, 2])
+    its = []
+
+    def binv(x):
+        y, _, _, it = P.fncinvB(x, 1e-12, 10000)
+        its.append(it)
+        if len(its) % 20 == 0:
+            log(f"config1: CPU baseline {len(its)} inner solves, {time.perf_counter() - t0:.0f} s")
+        return y
+
+    _, ew, steps = fo.lanczos(P.fncA, P.fncB, binv, 4 * P.N, nev, tol, 10000, p, np.ones(4 * P.N, complex))
+    return time.perf_counter() - t0, np.sort(1.0 / np.real(ew)), int(steps), float(np.mean(its)), fo.FFT_WORKERS
+
+
+def run_config1(pkg, torch, args):
+    """The reference's CPU-runnable case: SC woodpile n = 24, eps = 13, gamma = 0.2, one k-point
+    pi (0.5, 0.3, 0.1), the 10 smallest omega by Lanczos (p = 20, tol 1e-10, CG tol 1e-12,
+    v0 = ones).  GPU solve (table setup through converged pairs, plus restoration and
+    residuals) against the oracle's CPU restatement of the same algorithm on the host cores,
+    eigenvalues compared."""
+    S = pkg.structures
+    n = args.n
+    dims = (n, n, n)
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    table = S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE)
+    ids = S.woodpile_sc_ids(dims)
+    nev, p = 10, 20
+    pl = pkg.Plan(dims, theta, table, ids)  # warm-up (module load, kernel attributes, first torch calls)
+    ev, ew, _ = pl.Lanczos(2, tol=1e-6, maxiter=50, p=4)
+    pl.EvaluteError_Chiral_no_inv(ev, 1.0 / ew, pl.Eigen_Restoration_Chiral_no_inv(ev))
+    pl.close()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pl = pkg.Plan(dims, theta, table, ids)
+    ev, ew, it = pl.Lanczos(nev, tol=1e-10, maxiter=10000, p=p, cg_tol=1e-12, cg_maxit=10000)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    fields = pl.Eigen_Restoration_Chiral_no_inv(ev)
+    r_nf, r_org = pl.EvaluteError_Chiral_no_inv(ev, 1.0 / ew, fields)
+    torch.cuda.synchronize()
+    gpu_full = time.perf_counter() - t0
+    nx = torch.linalg.vector_norm(ev, dim=2).cpu().numpy()
+    nf = torch.linalg.vector_norm(fields, dim=2).cpu().numpy()
+    pl.close()
+    omega = np.sort(1.0 / ew[0])
+    log(f"config1: GPU solve {gpu_s:.2f} s ({int(it[0])} Lanczos steps), with restoration {gpu_full:.2f} s, "
+        f"omega[:4]={omega[:4]}")
+    out = {"metric": "s per k-point, n=24 woodpile, one k (the reference's CPU-runnable case)", "value": gpu_s,
+           "unit": "s/k-point", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+           "vs_baseline": None,
+           "config": {"workload": f"SC woodpile n={n}, eps=13, gamma=0.2, k=pi(0.5,0.3,0.1), {nev} smallest omega, "
+                                  f"Lanczos p={p} tol 1e-10, CG tol 1e-12, v0=ones"},
+           "s_with_restoration": gpu_full, "lanczos_steps": int(it[0]), "omega": omega.tolist(),
+           "max_relative_residual": {"nfsep": float((r_nf / nx).max()), "original": float((r_org / nf).max())}}
+    if not args.no_cpu:
+        log("config1: CPU baseline (the oracle's Lanczos on the host cores)")
+        cpu_s, omega_cpu, steps_cpu, cg_mean, workers = cpu_baseline_solve(n, theta, table, ids, nev, 1e-10, p)
+        out["cpu_baseline"] = {"value": cpu_s, "unit": "s/k-point", "cores": workers, "kind": "port",
+                               "sample": f"the full solve: oracle Lanczos (CPU restatement of Lanczos.m, pcg, fncB), "
+                                         f"{steps_cpu} Lanczos steps, {cg_mean:.1f} CG iterations per inner solve"}
+        out["omega_max_rel_diff_vs_oracle"] = float(np.max(np.abs(omega - omega_cpu) / omega_cpu))
+        log(f"config1: CPU {cpu_s:.1f} s; max |omega_gpu - omega_cpu| / omega = {out['omega_max_rel_diff_vs_oracle']:.2e}")
+    print(json.dumps(out), flush=True)
+
 
 def run_config3(pkg, torch, args):
     """64 k-points over three FCC sphere crystals (r = 0.12a, eps = 13, gamma ~ U[0, 3.4] from
@@ -425,9 +507,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ops", action="store_true", help="skip the per-operator rates")
-    ap.add_argument("--config", type=int, default=2, choices=[2, 3, 4, 5],
-                    help="2: the 40-k woodpile band path (default); 3: multi-grid FCC spheres; "
-                         "4: JD-SIRA near the Pasteur limit; 5: adjoint gradients")
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
+                    help="1: n=24 one-k solve against the CPU oracle; 2: the 40-k woodpile band path (default); "
+                         "3: multi-grid FCC spheres; 4: JD-SIRA near the Pasteur limit; 5: adjoint gradients")
     ap.add_argument("--concurrent", action="store_true",
                     help="configs 3 and 4: run the three solves at once on their own streams")
     ap.add_argument("--target", type=float, default=0.3, help="JD-SIRA target (config 4)")
@@ -455,9 +537,11 @@ def main():
         torch.cuda.set_device(0)
         if args.config == 5 and args.n == 60:
             args.n = 48
+        if args.config == 1 and args.n == 60:
+            args.n = 24
         if args.config == 4 and args.nev == 10:
             args.nev = 20
-        {3: run_config3, 4: run_config4, 5: run_config5}[args.config](pkg, torch, args)
+        {1: run_config1, 3: run_config3, 4: run_config4, 5: run_config5}[args.config](pkg, torch, args)
         return
     S = pkg.structures
     n, nk = args.n, args.nk
