@@ -7,8 +7,8 @@ outside, n = 60 Yee grid (4N = 864 000 complex128 unknowns per NFSEP vector),
 40 k-points on Gamma-X-M-R-Gamma, FP64.
 
 One step = one fncB (curl-SVD + material sandwich, 5 HIP kernels) applied to
-all 40 k-points of the band path in one batched launch sequence, inputs
-resident in HBM.  ``value`` = matvecs/s over the whole job (40 per step).
+all 40 k-points of the band path in one batched call (launches of an
+autotuned k-point chunk), inputs resident in HBM.  ``value`` = matvecs/s over the whole job (40 per step).
 Also reported: the plain per-k rate (one k-point per launch sequence), the
 batched/plain ratio, seconds per k-point of the full Lanczos solve (plain and
 batched), per-kernel HIP-event timings, the roofline of the dominant kernel,
