@@ -85,7 +85,8 @@ struct fame_plan {
   double2* d_w1 = nullptr;    // [nb][6N]
   double2* d_w2 = nullptr;    // [nb][6N]
   double2* d_part = nullptr;  // reduction partials
-  double2* d_part2 = nullptr; // fused-CG p^H q partials from the MIX_OUT pass
+  double2* d_part2 = nullptr; // fused-CG p^H q partials (material stencil or MIX_OUT pass)
+  size_t part2_cap = 0;       // elements of d_part2
   double2* d_mr = nullptr;    // MINRES work vectors [6][nb][4N], allocated by the first fame_minres
   void* d_mrst = nullptr;     // MINRES per-item state [nb]
   double2* d_mgs = nullptr;   // fame_mgs coefficients when the caller passes none
@@ -423,23 +424,33 @@ void material_tile(int n1, int n2, int n3, int nbc, int& TX, int& TY, int& TZ) {
 
 int pv_material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs);
 
-int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs) {
+// tile, slab and grid of the 2.5-D material kernel for a launch of cnb items
+void material_shape(const fame_plan* P, int cnb, int& TX, int& TY, int& TZ, int& ntile, int& nzc) {
+  material_tile(P->n1, P->n2, P->n3, cnb, TX, TY, TZ);
+  if (P->mt[0] > 0) {  // tile chosen by the autotuner or the caller
+    TX = P->mt[0];
+    TY = P->mt[1];
+    if (P->mt[2] > 0) TZ = P->mt[2];
+  }
+  ntile = ((P->n1 + TX - 1) / TX) * ((P->n2 + TY - 1) / TY);
+  nzc = (P->n3 + TZ - 1) / TZ;
+}
+
+// dpart (table-id stencil only): per-work-group sums of conj(in) . out, item stride pstride
+int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs, double2* dpart = nullptr,
+             int pstride = 0) {
   if (P->pv) return pv_material(P, in, ibs, out, obs);
   if (P->fast) {
-    int TX, TY, TZ;
-    material_tile(P->n1, P->n2, P->n3, P->cnb, TX, TY, TZ);
-    if (P->mt[0] > 0) {  // tile chosen by the autotuner or the caller
-      TX = P->mt[0];
-      TY = P->mt[1];
-      if (P->mt[2] > 0) TZ = P->mt[2];
-    }
-    const int ntile = ((P->n1 + TX - 1) / TX) * ((P->n2 + TY - 1) / TY);
-    const int nzc = (P->n3 + TZ - 1) / TZ;
+    int TX, TY, TZ, ntile, nzc;
+    material_shape(P, P->cnb, TX, TY, TZ, ntile, nzc);
+    if (dpart && ntile * nzc > pstride) return set_err(FAME_ERR_ARG, "material: partial-sum stride too small");
     hipLaunchKernelGGL(k_material_z, dim3(ntile, nzc, P->cnb), dim3(kMzThreads), material_z_lds(TX, TY), P->stream,
-                       P->geo, in, ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, wom(P), TX, TY, TZ);
+                       P->geo, in, ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, wom(P), TX, TY, TZ, dpart,
+                       pstride);
     CHECK_LAUNCH();
     return FAME_OK;
   }
+  if (dpart) return set_err(FAME_ERR_ARG, "material: partial sums need the specialised stencil");
   const int tx = (P->n1 + kMatTX - 1) / kMatTX, ty = (P->n2 + kMatTY - 1) / kMatTY;
   const int tz = (P->n3 + kMatTZ - 1) / kMatTZ;
   hipLaunchKernelGGL(k_material, dim3(tx * ty, tz, P->cnb), dim3(kBlk), 0, P->stream, P->geo, in, ibs, out, obs,
@@ -479,15 +490,17 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   }
   const bool pr = P->prof;
   if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
+  // CG: p^H q is summed by the table-id stencil (in real space) or else by the MIX_OUT pass
+  const bool mat_dot = pcf && !P->pv;
   RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0, pcf));
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
   RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
-  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N));
+  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N, mat_dot ? wcf.part : nullptr, mat_dot ? wcf.nparts : 0));
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
   RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
-  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, pcf));
+  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, mat_dot ? nullptr : pcf));
   if (pr) {
     HIP_TRY(hipEventRecord(P->ev[5], P->stream));
     HIP_TRY(hipEventSynchronize(P->ev[5]));
@@ -614,8 +627,24 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
   const long long L = 4 * P->N;
   const long long tot = L * P->vnb;
   const int nb = P->vnb;
-  const int nparts = 2 * ((P->geo.n12 + zfast_tp(P) - 1) / zfast_tp(P));
-  if (!P->d_part2) HIP_TRY(hipMalloc(&P->d_part2, (size_t)P->nb * 2 * ((P->geo.n12 + 7) / 8) * sizeof(double2)));
+  // p^H q partial sums per item: from the material stencil (one per work-group of a launch of
+  // nbc items) or, for per-vertex media, from the MIX_OUT pass (one per z work-group and slot)
+  const bool mat_dot = !P->pv;
+  auto nparts_of = [&](int nbc) {
+    if (!mat_dot) return 2 * ((P->geo.n12 + zfast_tp(P) - 1) / zfast_tp(P));
+    int TX, TY, TZ, ntile, nzc;
+    material_shape(P, nbc, TX, TY, TZ, ntile, nzc);
+    return ntile * nzc;
+  };
+  int nparts = 0;  // per-item stride of the partial sums: the largest count of any launch size
+  for (int c = 1; c <= nb; ++c) nparts = std::max(nparts, nparts_of(c));
+  if (P->part2_cap < (size_t)P->nb * nparts) {
+    if (P->d_part2) HIP_TRY(hipFree(P->d_part2));
+    P->d_part2 = nullptr;
+    P->part2_cap = 0;
+    HIP_TRY(hipMalloc(&P->d_part2, (size_t)P->nb * nparts * sizeof(double2)));
+    P->part2_cap = (size_t)P->nb * nparts;
+  }
   RET_IF(fill_zero(P, x, tot));
   RET_IF(copy(P, b, P->d_r, tot));
   RET_IF(fill_zero(P, P->d_p, tot));
@@ -628,8 +657,8 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
   // one iteration on the chunk [b0, b0 + nbc)
   auto step = [&](int b0, int nbc) -> int {
     RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_q, &cf));
-    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts, nparts,
-                       P->d_cg + b0);
+    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts,
+                       nparts_of(nbc), nparts, P->d_cg + b0);
     hipLaunchKernelGGL(k_cg_update_r, dim3(nblk, nbc), dim3(kBlk), 0, P->stream, P->d_r + (size_t)b0 * L,
                        P->d_q + (size_t)b0 * L, L, P->d_cg + b0, P->d_part + (size_t)b0 * nblk);
     hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part + (size_t)b0 * nblk, nblk,

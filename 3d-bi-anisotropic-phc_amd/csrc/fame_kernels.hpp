@@ -553,12 +553,12 @@ __global__ void k_cg_p(double2* __restrict__ p, const double2* __restrict__ r, l
 
 // Fused-CG scalar and vector steps (the p / x updates and the p^H q partial sums run
 // inside the first and last pass of fncB):
-// alpha = rho / Re(p^H q) from the partial sums of the MIX_OUT pass
-__global__ void __launch_bounds__(256) k_cg_alpha2(const double2* __restrict__ part, int nparts,
+// alpha = rho / Re(p^H q) from nparts partial sums per item (item stride `stride`)
+__global__ void __launch_bounds__(256) k_cg_alpha2(const double2* __restrict__ part, int nparts, int stride,
                                                    CgState* __restrict__ st) {
   const int b = blockIdx.x;
   double2 acc = make_double2(0, 0);
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = cadd(acc, part[(long long)b * nparts + i]);
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = cadd(acc, part[(long long)b * stride + i]);
   acc = block_sum(acc);
   if (threadIdx.x == 0) {
     CgState s = st[b];

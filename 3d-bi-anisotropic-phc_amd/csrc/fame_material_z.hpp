@@ -6,7 +6,9 @@
 //   * x/y neighbours are exchanged through small per-slice LDS arrays,
 // and every field value is fetched from HBM once (plus the one-column halo).
 // The arithmetic is the vertex form of k_material (same gathers, 6x6 vertex
-// products and scatters).
+// products and scatters).  With dpart, each work-group also leaves the sum over its
+// outputs of conj(input) . output: summed over the grid this is u^H (Mat u), which for
+// u = the transformed CG direction p equals p^H fncB p (the transforms are unitary).
 #pragma once
 #include "fame_kernels.hpp"
 
@@ -31,7 +33,8 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
                                                            double2* __restrict__ fout, long long out_bstride,
                                                            const unsigned char* __restrict__ mat_id,
                                                            const double2* __restrict__ mat, int n_mat,
-                                                           const double2* __restrict__ omega, int TX, int TY, int TZ) {
+                                                           const double2* __restrict__ omega, int TX, int TY, int TZ,
+                                                           double2* __restrict__ dpart, int pstride) {
   extern __shared__ double2 sm[];
   const int RFX = TX + 2, RFY = TY + 2, RF = RFX * RFY;
   double2* tab = sm;
@@ -134,6 +137,7 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
     products(0, vez, id0, hold2, hold3, hold4, cur5);
   }
   int buf = 0;
+  double2 dacc = z2;  // sum of conj(input) . output over this thread's points (dpart only)
   double2 ne0 = z2, ne1 = z2, ne2 = z2, nh0 = z2, nh1 = z2, nh2 = z2;
   int id_n = 0;
   load((z0 + 1) % n3, ne0, ne1, ne2, nh0, nh1, nh2, id_n);
@@ -169,9 +173,13 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
         const double2 y0 = cscale(cfma(W234[RF + t], om0, W234[RF + t + 1]), 0.5);
         const double2 y1 = cscale(cfma(W234[2 * RF + t], om1, W234[2 * RF + t + RFX]), 0.5);
         const double2 y2 = cscale(cfma(cur5, om2, nxt5), 0.5);
-        EO[p] = make_double2(y0.x + d0 * e0.x, y0.y + d0 * e0.y);
-        EO[N + p] = make_double2(y1.x + d1 * e1.x, y1.y + d1 * e1.y);
-        EO[2 * N + p] = make_double2(y2.x + d2 * e2.x, y2.y + d2 * e2.y);
+        const double2 o0 = make_double2(y0.x + d0 * e0.x, y0.y + d0 * e0.y);
+        const double2 o1 = make_double2(y1.x + d1 * e1.x, y1.y + d1 * e1.y);
+        const double2 o2 = make_double2(y2.x + d2 * e2.x, y2.y + d2 * e2.y);
+        EO[p] = o0;
+        EO[N + p] = o1;
+        EO[2 * N + p] = o2;
+        if (dpart) dacc = cfmac(cfmac(cfmac(dacc, e0, o0), e1, o1), e2, o2);
       }
       {  // faces: h_x over (y, z), h_y over (x, z), h_z over (x, y)
         const double d0 = 0.25 * (mud(i00, 0) + mud(i01, 0) + mud(j00, 0) + mud(j01, 0));
@@ -186,9 +194,13 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
         a = cfma(W234[t], om0, W234[t + 1]);
         c = cfma(W234[t + RFX], om0, W234[t + RFX + 1]);
         const double2 y2 = cscale(cfma(a, om1, c), 0.25);
-        HO[p] = make_double2(y0.x + d0 * h0.x, y0.y + d0 * h0.y);
-        HO[N + p] = make_double2(y1.x + d1 * h1.x, y1.y + d1 * h1.y);
-        HO[2 * N + p] = make_double2(y2.x + d2 * h2.x, y2.y + d2 * h2.y);
+        const double2 o0 = make_double2(y0.x + d0 * h0.x, y0.y + d0 * h0.y);
+        const double2 o1 = make_double2(y1.x + d1 * h1.x, y1.y + d1 * h1.y);
+        const double2 o2 = make_double2(y2.x + d2 * h2.x, y2.y + d2 * h2.y);
+        HO[p] = o0;
+        HO[N + p] = o1;
+        HO[2 * N + p] = o2;
+        if (dpart) dacc = cfmac(cfmac(cfmac(dacc, h0, o0), h1, o1), h2, o2);
       }
     }
     // advance: slice s+1 becomes current
@@ -199,6 +211,10 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
     id_n = id_f;
     cur5 = nxt5;
     buf ^= 1;
+  }
+  if (dpart) {  // uniform over the grid: one partial sum per work-group, [item][slab][tile]
+    dacc = block_sum(dacc);
+    if (threadIdx.x == 0) dpart[(long long)b * pstride + zc * gridDim.x + tile] = dacc;
   }
 }
 
