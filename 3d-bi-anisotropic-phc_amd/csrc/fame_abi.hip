@@ -272,7 +272,7 @@ int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long 
                   int nslot, int grp_base, double sc, const CgFuse* cf) {
   const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->cnb);
   const size_t lds = (size_t)(3 * N * kZTP + N) * sizeof(double2);
-  const CgFuse c0{nullptr, nullptr, nullptr, nullptr, 0};
+  const CgFuse c0{nullptr, nullptr, nullptr, nullptr, 0, 0};
   if (dir > 0) {
     if (mode == ZM_MIX_IN && cf)
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT, 1>), grid, dim3(NT), lds, P->stream, P->geo, in,
@@ -284,7 +284,10 @@ int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long 
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
                          out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, c0);
   } else {
-    if (mode == ZM_MIX_OUT && cf)
+    if (mode == ZM_MIX_OUT && cf && cf->mode == 2)
+      hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT, 2>), grid, dim3(NT), lds, P->stream, P->geo, in,
+                         ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, *cf);
+    else if (mode == ZM_MIX_OUT && cf)
       hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT, 1>), grid, dim3(NT), lds, P->stream, P->geo, in,
                          ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, *cf);
     else if (mode == ZM_MIX_OUT)
@@ -317,6 +320,7 @@ int z_attrs() {
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_PLAIN, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_MIX_IN, TP, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, TP, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
+  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, TP, NT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   return FAME_OK;
 }
 
@@ -490,17 +494,25 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   }
   const bool pr = P->prof;
   if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
-  // CG: p^H q is summed by the table-id stencil (in real space) or else by the MIX_OUT pass
+  // CG: p^H q is summed by the table-id stencil (in real space), alpha formed right after it,
+  // and the MIX_OUT pass updates r (y is r then); per-vertex media sum p^H q in MIX_OUT instead
   const bool mat_dot = pcf && !P->pv;
   RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0, pcf));
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
   RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
   RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N, mat_dot ? wcf.part : nullptr, mat_dot ? wcf.nparts : 0));
+  if (mat_dot) {
+    int TX, TY, TZ, ntile, nzc;
+    material_shape(P, nbc, TX, TY, TZ, ntile, nzc);
+    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, ntile * nzc, wcf.nparts, wcf.st);
+    CHECK_LAUNCH();
+    wcf.mode = 2;
+  }
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
   RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
-  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, mat_dot ? nullptr : pcf));
+  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, pcf));
   if (pr) {
     HIP_TRY(hipEventRecord(P->ev[5], P->stream));
     HIP_TRY(hipEventSynchronize(P->ev[5]));
@@ -629,14 +641,16 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
   const int nb = P->vnb;
   // p^H q partial sums per item: from the material stencil (one per work-group of a launch of
   // nbc items) or, for per-vertex media, from the MIX_OUT pass (one per z work-group and slot)
+  // (with the stencil's sums, the MIX_OUT pass then leaves ||r||^2 partials in the same slots)
   const bool mat_dot = !P->pv;
+  const int nparts_z = 2 * ((P->geo.n12 + zfast_tp(P) - 1) / zfast_tp(P));
   auto nparts_of = [&](int nbc) {
-    if (!mat_dot) return 2 * ((P->geo.n12 + zfast_tp(P) - 1) / zfast_tp(P));
+    if (!mat_dot) return nparts_z;
     int TX, TY, TZ, ntile, nzc;
     material_shape(P, nbc, TX, TY, TZ, ntile, nzc);
     return ntile * nzc;
   };
-  int nparts = 0;  // per-item stride of the partial sums: the largest count of any launch size
+  int nparts = nparts_z;  // per-item stride of the partial sums: the largest count of any launch
   for (int c = 1; c <= nb; ++c) nparts = std::max(nparts, nparts_of(c));
   if (P->part2_cap < (size_t)P->nb * nparts) {
     if (P->d_part2) HIP_TRY(hipFree(P->d_part2));
@@ -651,18 +665,24 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
   RET_IF(do_dot(P, b, L, b, L, L, nb, P->d_red));
   hipLaunchKernelGGL(k_cg_init, dim3((nb + 63) / 64), dim3(64), 0, P->stream, P->d_red, P->d_cg, nb, tol);
   CHECK_LAUNCH();
-  const CgFuse cf{P->d_cg, x, P->d_p, P->d_part2, nparts};
+  const CgFuse cf{P->d_cg, x, P->d_p, P->d_part2, nparts, 1};
   const int nblk = std::min<int>(P->nblk_red, (int)grid1d(L));
   std::vector<CgState> hs(nb);
   // one iteration on the chunk [b0, b0 + nbc)
   auto step = [&](int b0, int nbc) -> int {
-    RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_q, &cf));
-    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts,
-                       nparts_of(nbc), nparts, P->d_cg + b0);
-    hipLaunchKernelGGL(k_cg_update_r, dim3(nblk, nbc), dim3(kBlk), 0, P->stream, P->d_r + (size_t)b0 * L,
-                       P->d_q + (size_t)b0 * L, L, P->d_cg + b0, P->d_part + (size_t)b0 * nblk);
-    hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part + (size_t)b0 * nblk, nblk,
-                       P->d_cg + b0);
+    if (mat_dot) {  // alpha and r <- r - alpha q inside the pass chain; then beta from ||r||^2
+      RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_r, &cf));
+      hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts, nparts_z,
+                         nparts, P->d_cg + b0);
+    } else {
+      RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_q, &cf));
+      hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts,
+                         nparts_z, nparts, P->d_cg + b0);
+      hipLaunchKernelGGL(k_cg_update_r, dim3(nblk, nbc), dim3(kBlk), 0, P->stream, P->d_r + (size_t)b0 * L,
+                         P->d_q + (size_t)b0 * L, L, P->d_cg + b0, P->d_part + (size_t)b0 * nblk);
+      hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part + (size_t)b0 * nblk, nblk, nblk,
+                         P->d_cg + b0);
+    }
     CHECK_LAUNCH();
     return FAME_OK;
   };
@@ -766,7 +786,7 @@ int do_cg_plain(fame_plan* P, const double2* b, double2* x, double tol, int maxi
     hipLaunchKernelGGL(k_cg_alpha, dim3((nb + 63) / 64), dim3(64), 0, P->stream, P->d_red, P->d_cg, nb);
     hipLaunchKernelGGL(k_cg_update, dim3(nblk, nb), dim3(kBlk), 0, P->stream, x, P->d_r, P->d_p, P->d_q, L, P->d_cg,
                        P->d_part);
-    hipLaunchKernelGGL(k_cg_beta, dim3(nb), dim3(kBlk), 0, P->stream, P->d_part, nblk, P->d_cg);
+    hipLaunchKernelGGL(k_cg_beta, dim3(nb), dim3(kBlk), 0, P->stream, P->d_part, nblk, nblk, P->d_cg);
     hipLaunchKernelGGL(k_cg_p, dim3(grid1d(L), nb), dim3(kBlk), 0, P->stream, P->d_p, P->d_r, L, P->d_cg);
     CHECK_LAUNCH();
     ++it;

@@ -515,10 +515,12 @@ __global__ void __launch_bounds__(256) k_cg_update(double2* __restrict__ x, doub
 }
 
 // reduce ||r||^2, then beta, rho, convergence flag (one block per batch item)
-__global__ void __launch_bounds__(256) k_cg_beta(const double2* __restrict__ part, int nblk, CgState* __restrict__ st) {
+// nblk partial sums of ||r||^2 per item, item stride `stride`
+__global__ void __launch_bounds__(256) k_cg_beta(const double2* __restrict__ part, int nblk, int stride,
+                                                 CgState* __restrict__ st) {
   const int b = blockIdx.x;
   double2 acc = make_double2(0, 0);
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) acc = cadd(acc, part[b * nblk + i]);
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) acc = cadd(acc, part[(long long)b * stride + i]);
   acc = block_sum(acc);
   if (threadIdx.x == 0) {
     CgState s = st[b];
