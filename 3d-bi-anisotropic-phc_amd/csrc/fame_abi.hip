@@ -272,7 +272,7 @@ int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long 
                   int nslot, int grp_base, double sc, const CgFuse* cf) {
   const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->cnb);
   const size_t lds = (size_t)(3 * N * kZTP + N) * sizeof(double2);
-  const CgFuse c0{nullptr, nullptr, nullptr, nullptr, 0, 0};
+  const CgFuse c0{nullptr, nullptr, nullptr, nullptr, 0};
   if (dir > 0) {
     if (mode == ZM_MIX_IN && cf)
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT, 1>), grid, dim3(NT), lds, P->stream, P->geo, in,
@@ -284,10 +284,7 @@ int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long 
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_PLAIN, kZTP, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs,
                          out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, c0);
   } else {
-    if (mode == ZM_MIX_OUT && cf && cf->mode == 2)
-      hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT, 2>), grid, dim3(NT), lds, P->stream, P->geo, in,
-                         ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, *cf);
-    else if (mode == ZM_MIX_OUT && cf)
+    if (mode == ZM_MIX_OUT && cf)
       hipLaunchKernelGGL((k_zfast<N, A, B, -1, ZM_MIX_OUT, kZTP, NT, 1>), grid, dim3(NT), lds, P->stream, P->geo, in,
                          ibs, out, obs, wlam(P), P->tab_len, P->d_tw, sc, grp_base, *cf);
     else if (mode == ZM_MIX_OUT)
@@ -320,7 +317,6 @@ int z_attrs() {
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_PLAIN, TP, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, 1, ZM_MIX_IN, TP, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, TP, NT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
-  HIP_TRY(hipFuncSetAttribute((const void*)k_zfast<N, A, B, -1, ZM_MIX_OUT, TP, NT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lz));
   return FAME_OK;
 }
 
@@ -426,7 +422,8 @@ void material_tile(int n1, int n2, int n3, int nbc, int& TX, int& TY, int& TZ) {
   TZ = (int)((n3 + nzc - 1) / nzc);
 }
 
-int pv_material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs);
+int pv_material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs, double2* dpart,
+                int pstride);
 
 // tile, slab and grid of the 2.5-D material kernel for a launch of cnb items
 void material_shape(const fame_plan* P, int cnb, int& TX, int& TY, int& TZ, int& ntile, int& nzc) {
@@ -440,10 +437,19 @@ void material_shape(const fame_plan* P, int cnb, int& TX, int& TY, int& TZ, int&
   nzc = (P->n3 + TZ - 1) / TZ;
 }
 
-// dpart (table-id stencil only): per-work-group sums of conj(in) . out, item stride pstride
+// number of per-work-group partial sums the material pass leaves per item (dpart below)
+int material_parts(const fame_plan* P, int cnb) {
+  if (P->pv) return (int)grid1d(P->N);
+  int TX, TY, TZ, ntile, nzc;
+  material_shape(P, cnb, TX, TY, TZ, ntile, nzc);
+  return ntile * nzc;
+}
+
+// dpart (specialised and per-vertex stencils): per-work-group sums of conj(in) . out, item
+// stride pstride
 int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs, double2* dpart = nullptr,
              int pstride = 0) {
-  if (P->pv) return pv_material(P, in, ibs, out, obs);
+  if (P->pv) return pv_material(P, in, ibs, out, obs, dpart, pstride);
   if (P->fast) {
     int TX, TY, TZ, ntile, nzc;
     material_shape(P, P->cnb, TX, TY, TZ, ntile, nzc);
@@ -494,20 +500,17 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   }
   const bool pr = P->prof;
   if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
-  // CG: p^H q is summed by the table-id stencil (in real space), alpha formed right after it,
-  // and the MIX_OUT pass updates r (y is r then); per-vertex media sum p^H q in MIX_OUT instead
-  const bool mat_dot = pcf && !P->pv;
+  // CG: the material pass sums p^H q (in real space), alpha is formed right after it, and the
+  // MIX_OUT pass updates r in place (y is r then)
   RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0, pcf));
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
   RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
-  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N, mat_dot ? wcf.part : nullptr, mat_dot ? wcf.nparts : 0));
-  if (mat_dot) {
-    int TX, TY, TZ, ntile, nzc;
-    material_shape(P, nbc, TX, TY, TZ, ntile, nzc);
-    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, ntile * nzc, wcf.nparts, wcf.st);
+  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N, pcf ? wcf.part : nullptr, pcf ? wcf.nparts : 0));
+  if (pcf) {
+    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, material_parts(P, nbc), wcf.nparts,
+                       wcf.st);
     CHECK_LAUNCH();
-    wcf.mode = 2;
   }
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
   RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
@@ -639,19 +642,11 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
   const long long L = 4 * P->N;
   const long long tot = L * P->vnb;
   const int nb = P->vnb;
-  // p^H q partial sums per item: from the material stencil (one per work-group of a launch of
-  // nbc items) or, for per-vertex media, from the MIX_OUT pass (one per z work-group and slot)
-  // (with the stencil's sums, the MIX_OUT pass then leaves ||r||^2 partials in the same slots)
-  const bool mat_dot = !P->pv;
+  // per-item partial sums: p^H q from the material pass (one per work-group of a launch of nbc
+  // items), then ||r||^2 from the MIX_OUT pass (one per z work-group and slot) in the same slots
   const int nparts_z = 2 * ((P->geo.n12 + zfast_tp(P) - 1) / zfast_tp(P));
-  auto nparts_of = [&](int nbc) {
-    if (!mat_dot) return nparts_z;
-    int TX, TY, TZ, ntile, nzc;
-    material_shape(P, nbc, TX, TY, TZ, ntile, nzc);
-    return ntile * nzc;
-  };
-  int nparts = nparts_z;  // per-item stride of the partial sums: the largest count of any launch
-  for (int c = 1; c <= nb; ++c) nparts = std::max(nparts, nparts_of(c));
+  int nparts = nparts_z;  // per-item stride: the largest count of any launch
+  for (int c = 1; c <= nb; ++c) nparts = std::max(nparts, material_parts(P, c));
   if (P->part2_cap < (size_t)P->nb * nparts) {
     if (P->d_part2) HIP_TRY(hipFree(P->d_part2));
     P->d_part2 = nullptr;
@@ -665,24 +660,14 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
   RET_IF(do_dot(P, b, L, b, L, L, nb, P->d_red));
   hipLaunchKernelGGL(k_cg_init, dim3((nb + 63) / 64), dim3(64), 0, P->stream, P->d_red, P->d_cg, nb, tol);
   CHECK_LAUNCH();
-  const CgFuse cf{P->d_cg, x, P->d_p, P->d_part2, nparts, 1};
-  const int nblk = std::min<int>(P->nblk_red, (int)grid1d(L));
+  const CgFuse cf{P->d_cg, x, P->d_p, P->d_part2, nparts};
   std::vector<CgState> hs(nb);
-  // one iteration on the chunk [b0, b0 + nbc)
+  // one iteration on the chunk [b0, b0 + nbc): alpha and r <- r - alpha q inside the pass
+  // chain (the output of fncB is r itself), then beta from ||r||^2
   auto step = [&](int b0, int nbc) -> int {
-    if (mat_dot) {  // alpha and r <- r - alpha q inside the pass chain; then beta from ||r||^2
-      RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_r, &cf));
-      hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts, nparts_z,
-                         nparts, P->d_cg + b0);
-    } else {
-      RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_q, &cf));
-      hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts,
-                         nparts_z, nparts, P->d_cg + b0);
-      hipLaunchKernelGGL(k_cg_update_r, dim3(nblk, nbc), dim3(kBlk), 0, P->stream, P->d_r + (size_t)b0 * L,
-                         P->d_q + (size_t)b0 * L, L, P->d_cg + b0, P->d_part + (size_t)b0 * nblk);
-      hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part + (size_t)b0 * nblk, nblk, nblk,
-                         P->d_cg + b0);
-    }
+    RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_r, &cf));
+    hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts, nparts_z,
+                       nparts, P->d_cg + b0);
     CHECK_LAUNCH();
     return FAME_OK;
   };

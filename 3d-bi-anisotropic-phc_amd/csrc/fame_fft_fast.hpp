@@ -173,18 +173,16 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
 // CG = 1 fuses the conjugate-gradient vector updates into the first and last pass of fncB:
 //   MIX_IN  reads r (the input), p and x at each point, forms p <- r + beta p and the
 //           deferred x <- x + alpha p_old, writes p and x back and transforms p;
-//   MIX_OUT accumulates conj(p) . q over its outputs q and leaves one partial sum per
-//           (item, slot, work-group) in cf.part[item * cf.nparts + slot * gridDim.x + blockIdx.x].
-// CG = 2 (MIX_OUT, when alpha is already known from p^H q summed earlier in the pass chain):
-//   the output array holds r; each output q updates it in place, r <- r - alpha q (q is not
-//   stored), and the partial sums are of ||r||^2 (in .x), at the same positions.
+//   MIX_OUT (alpha is known: p^H q was summed by the material pass) holds r in its output
+//           array and updates it in place from each output q, r <- r - alpha q (q is not
+//           stored), leaving one partial sum of ||r||^2 (in .x) per (item, slot, work-group)
+//           in cf.part[item * cf.nparts + slot * gridDim.x + blockIdx.x].
 struct CgFuse {
   CgState* st;
   double2* x;
   double2* p;
   double2* part;
   int nparts;
-  int mode;  // MIX_OUT: 1 = p^H q partials, 2 = r update and ||r||^2 partials
 };
 
 template <int N3, int A, int B, int DIR, int MODE, int TP, int NT, int CG = 0>
@@ -325,8 +323,7 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
     }
     __syncthreads();
     double2* __restrict__ yout = out + b * out_bstride + (long long)(2 * slot) * N;
-    const double2* __restrict__ pq = CG == 1 ? cf.p + b * out_bstride + (long long)(2 * slot) * N : nullptr;
-    const double ca = CG == 2 ? cf.st[b].alpha : 0.0;
+    const double ca = CG ? cf.st[b].alpha : 0.0;
     double2 dacc = make_double2(0, 0);
     for (int it = threadIdx.x; it < N3 * TP; it += NT) {
       const int k3 = it / TP, pp = it - (it / TP) * TP;
@@ -347,7 +344,7 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
         o2 = cfmac(cfmac(cmulc(y0, p2[0]), p2[1], y1), p2[2], y2);
       }
       const long long P = q + (long long)n12 * k3;
-      if (CG == 2) {
+      if (CG) {
         double2 r1 = yout[P], r2 = yout[N + P];
         r1 = make_double2(fma(-ca, o1.x, r1.x), fma(-ca, o1.y, r1.y));
         r2 = make_double2(fma(-ca, o2.x, r2.x), fma(-ca, o2.y, r2.y));
@@ -358,7 +355,6 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
       } else {
         yout[P] = o1;
         yout[N + P] = o2;
-        if (CG == 1) dacc = cfmac(cfmac(dacc, pq[P], o1), pq[N + P], o2);
       }
     }
     if (CG) {

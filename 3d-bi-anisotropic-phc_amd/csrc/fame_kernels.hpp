@@ -553,8 +553,8 @@ __global__ void k_cg_p(double2* __restrict__ p, const double2* __restrict__ r, l
   }
 }
 
-// Fused-CG scalar and vector steps (the p / x updates and the p^H q partial sums run
-// inside the first and last pass of fncB):
+// Fused-CG scalar steps (the p / x updates run inside the first pass of fncB, the p^H q
+// partial sums inside the material pass and the r update inside the last pass):
 // alpha = rho / Re(p^H q) from nparts partial sums per item (item stride `stride`)
 __global__ void __launch_bounds__(256) k_cg_alpha2(const double2* __restrict__ part, int nparts, int stride,
                                                    CgState* __restrict__ st) {
@@ -567,26 +567,6 @@ __global__ void __launch_bounds__(256) k_cg_alpha2(const double2* __restrict__ p
     s.alpha = (s.active != 0.0) ? s.rho / acc.x : 0.0;
     st[b] = s;
   }
-}
-
-// r -= alpha q ; partial ||r||^2
-__global__ void __launch_bounds__(256) k_cg_update_r(double2* __restrict__ r, const double2* __restrict__ q,
-                                                     long long len, const CgState* __restrict__ st,
-                                                     double2* __restrict__ part) {
-  const int b = blockIdx.y;
-  const double a = st[b].alpha;
-  const long long off = b * len;
-  double acc = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x) {
-    const double2 qv = q[off + i];
-    double2 rv = r[off + i];
-    rv.x = fma(-a, qv.x, rv.x);
-    rv.y = fma(-a, qv.y, rv.y);
-    r[off + i] = rv;
-    acc = fma(rv.x, rv.x, fma(rv.y, rv.y, acc));
-  }
-  double2 sm = block_sum(make_double2(acc, 0));
-  if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = sm;
 }
 
 // the deferred last update x += alpha p
