@@ -720,7 +720,14 @@ def main():
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             pl = pkg.Plan(dims, thetas, table, ids)
-            pl.set_chunk(chunk)
+            if args.chunk < 0:
+                pl.autotune()  # chunk, launch shapes and material tile of the batched plan (timed)
+            else:              # the fixed configuration given on the command line
+                pl.set_chunk(chunk)
+                if args.variant:
+                    pl.set_variant(*args.variant)
+                if args.mtile:
+                    pl.set_material_tile(*args.mtile)
             ev, ew, it = pl.Lanczos(k, tol=1e-10, maxiter=10000, p=2 * k, cg_tol=1e-12, cg_maxit=10000)
             torch.cuda.synchronize()
             tb = time.perf_counter() - t0
