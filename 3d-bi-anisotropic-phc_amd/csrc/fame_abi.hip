@@ -979,7 +979,9 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   double2 *x = nullptr, *y = nullptr;
   HIP_TRY(hipMalloc(&x, bytes));
   HIP_TRY(hipMalloc(&y, bytes));
-  RET_IF(fill_zero(P, x, (long long)P->nb * 4 * P->N));
+  hipLaunchKernelGGL(k_fill_pattern, dim3(grid1d((long long)P->nb * 4 * P->N)), dim3(kBlk), 0, P->stream, x,
+                     (long long)P->nb * 4 * P->N);
+  CHECK_LAUNCH();
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));

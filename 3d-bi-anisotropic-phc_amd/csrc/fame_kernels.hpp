@@ -628,6 +628,18 @@ __global__ void k_twist(Geo g, const double2* __restrict__ x, double2* __restric
   }
 }
 
+// y[i] = a fixed pseudo-random pattern in [-0.5, 0.5)^2 (integer hash of i): timing input
+// whose data-dependent power draw is that of real fields, unlike zeros
+__global__ void k_fill_pattern(double2* __restrict__ y, long long len) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x) {
+    unsigned long long h = (unsigned long long)i * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 32;
+    y[i] = make_double2((double)(h & 0xFFFFFFull) / 16777216.0 - 0.5, (double)((h >> 24) & 0xFFFFFFull) / 16777216.0 - 0.5);
+  }
+}
+
 __global__ void k_fill_zero(double2* __restrict__ y, long long len) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)
     y[i] = make_double2(0, 0);
