@@ -127,9 +127,13 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
   double2 pe2 = z2, ph0 = z2, ph1 = z2;                           // previous slice partial inputs
   double2 hold2 = z2, hold3 = z2, hold4 = z2, cur5 = z2, nxt5 = z2;
   {
-    double2 a0, a1, c2;
-    int idd = 0, id0 = 0;
-    load((z0 - 1 + n3) % n3, a0, a1, pe2, ph0, ph1, c2, idd);
+    int id0 = 0;
+    if (act) {  // slice z0-1: only its e_z, h_x, h_y enter (the z-partials of slice z0)
+      const long long p = col + plane * ((z0 - 1 + n3) % n3);
+      pe2 = E[2 * N + p];
+      ph0 = H[p];
+      ph1 = H[N + p];
+    }
     load(z0, e0, e1, e2, h0, h1, h2, id0);
     put_slice(0, e0, e1, h0, h1, h2, pe2, ph0, ph1, id0);
     const double2 vez = cscale(cfmac(e2, om2, pe2), 0.5);
