@@ -976,16 +976,41 @@ int fame_plan_get_variant(fame_plan* P, int* xy_variant, int* z_variant) {
 int fame_plan_autotune(fame_plan* P, double* best_ms) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   const size_t bytes = (size_t)P->nb * 4 * P->N * sizeof(double2);
-  double2 *x = nullptr, *y = nullptr;
-  HIP_TRY(hipMalloc(&x, bytes));
-  HIP_TRY(hipMalloc(&y, bytes));
+  // on every exit: free the work buffers and events, restore profiling; on an error also
+  // restore the configuration the plan had before
+  struct Guard {
+    fame_plan* P;
+    double2 *x = nullptr, *y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool prof, ok = false;
+    int xyv, zv, chunk, mt[3];
+    explicit Guard(fame_plan* p)
+        : P(p), prof(p->prof), xyv(p->xyv), zv(p->zv), chunk(p->chunk), mt{p->mt[0], p->mt[1], p->mt[2]} {}
+    ~Guard() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (x || y) (void)hipStreamSynchronize(P->stream);
+      if (x) (void)hipFree(x);
+      if (y) (void)hipFree(y);
+      P->prof = prof;
+      if (!ok) {
+        P->xyv = xyv;
+        P->zv = zv;
+        P->chunk = chunk;
+        for (int i = 0; i < 3; ++i) P->mt[i] = mt[i];
+      }
+    }
+  } G(P);
+  HIP_TRY(hipMalloc(&G.x, bytes));
+  HIP_TRY(hipMalloc(&G.y, bytes));
+  double2* x = G.x;
+  double2* y = G.y;
   hipLaunchKernelGGL(k_fill_pattern, dim3(grid1d((long long)P->nb * 4 * P->N)), dim3(kBlk), 0, P->stream, x,
                      (long long)P->nb * 4 * P->N);
   CHECK_LAUNCH();
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  const bool prof = P->prof;
+  HIP_TRY(hipEventCreate(&G.e0));
+  HIP_TRY(hipEventCreate(&G.e1));
+  hipEvent_t e0 = G.e0, e1 = G.e1;
   P->prof = false;
   auto time_it = [&](double& ms_out) -> int {
     RET_IF(do_fncB(P, x, y));  // warm-up
@@ -1052,11 +1077,7 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
     P->mt[1] = bty;
     P->mt[2] = btz;
   }
-  P->prof = prof;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(x);
-  (void)hipFree(y);
+  G.ok = true;
   if (best_ms) *best_ms = best;
   return FAME_OK;
 }
