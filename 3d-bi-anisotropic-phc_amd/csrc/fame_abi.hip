@@ -125,7 +125,7 @@ struct fame_plan {
   double2* d_om_v = nullptr;
   int* d_idx = nullptr;       // [nb] item indices of the packed view
   int chunk = 0;              // fncB processes the batch in chunks of this many items (0: all)
-  int xyv = 0;                // xy-pass launch shape: 0 = default threads, 1 = 512 threads
+  int xyv = 0;                // xy-pass launch shape: 0 = default threads, 1 = 512 threads (default from n1 >= 40)
   int zv = 0;                 // z-pass launch shape: 0 = 16 pencils/256 thr, 1 = 8/128, 2 = 32/512
   // per-stage timing of fncB (HIP events between the five kernels), off by default
   bool prof = false;
@@ -416,9 +416,11 @@ void material_tile(int n1, int n2, int n3, int nbc, int& TX, int& TY, int& TZ) {
         TY = ty;
       }
     }
+  // slabs: about 2048 work-groups in all, slabs of at least 3 slices (each slab re-reads
+  // one slice below and one above)
   const long long tiles = (long long)((n1 + TX - 1) / TX) * ((n2 + TY - 1) / TY) * nbc;
   long long nzc = (2048 + tiles - 1) / tiles;
-  nzc = std::max<long long>(1, std::min<long long>(nzc, std::max(1, n3 / 4)));
+  nzc = std::max<long long>(1, std::min<long long>(nzc, std::max(1, n3 / 3)));
   TZ = (int)((n3 + nzc - 1) / nzc);
 }
 
@@ -817,6 +819,7 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   fame_plan* P = new fame_plan();
   P->n1 = n1; P->n2 = n2; P->n3 = n3; P->nb = nb; P->N = N; P->n_mat = n_mat;
   P->cb0 = 0; P->cnb = nb; P->vnb = nb;
+  P->xyv = n1 >= 40 ? 1 : 0;  // 512-thread xy kernels: faster from n = 48 up, batched or not
   P->tab_len = n1 + n2 + n3;
   P->theta.assign(theta, theta + 3 * nb);
   for (int d = 0; d < 3; ++d) P->delta[d] = delta[d];
