@@ -211,7 +211,7 @@ def cpu_baseline_solve(n, theta, table, ids, nev, tol, p):
 def run_config1(pkg, torch, args):
     """The reference's CPU-runnable case: SC woodpile n = 24, eps = 13, gamma = 0.2, one k-point
     pi (0.5, 0.3, 0.1), the 10 smallest omega by Lanczos (p = 20, tol 1e-10, CG tol 1e-12,
-    v0 = ones).  GPU solve (table setup through converged pairs, plus restoration and
+    v0 = ones as the reference; also from a complex-Gaussian start, default_rng(0)).  GPU solve (table setup through converged pairs, plus restoration and
     residuals) against the oracle's CPU restatement of the same algorithm on the host cores,
     eigenvalues compared."""
     S = pkg.structures
@@ -237,16 +237,28 @@ def run_config1(pkg, torch, args):
     gpu_full = time.perf_counter() - t0
     nx = torch.linalg.vector_norm(ev, dim=2).cpu().numpy()
     nf = torch.linalg.vector_norm(fields, dim=2).cpu().numpy()
+    # the complex-Gaussian start vector (numpy default_rng(0)) beside the reference's ones
+    rng0 = np.random.default_rng(0)
+    v0 = rng0.standard_normal(4 * pl.N) + 1j * rng0.standard_normal(4 * pl.N)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, ew_g, it_g = pl.Lanczos(nev, tol=1e-10, maxiter=10000, p=p, v0=v0, cg_tol=1e-12, cg_maxit=10000)
+    torch.cuda.synchronize()
+    gauss_s = time.perf_counter() - t1
     pl.close()
     omega = np.sort(1.0 / ew[0])
+    omega_g = np.sort(1.0 / ew_g[0])
     log(f"config1: GPU solve {gpu_s:.2f} s ({int(it[0])} Lanczos steps), with restoration {gpu_full:.2f} s, "
-        f"omega[:4]={omega[:4]}")
+        f"omega[:4]={omega[:4]}; Gaussian start {gauss_s:.2f} s ({int(it_g[0])} steps), "
+        f"max |d omega|/omega = {np.max(np.abs(omega_g - omega) / omega):.1e}")
     out = {"metric": "s per k-point, n=24 woodpile, one k (the reference's CPU-runnable case)", "value": gpu_s,
            "unit": "s/k-point", "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
            "vs_baseline": None,
            "config": {"workload": f"SC woodpile n={n}, eps=13, gamma=0.2, k=pi(0.5,0.3,0.1), {nev} smallest omega, "
                                   f"Lanczos p={p} tol 1e-10, CG tol 1e-12, v0=ones"},
            "s_with_restoration": gpu_full, "lanczos_steps": int(it[0]), "omega": omega.tolist(),
+           "gaussian_start": {"seconds": gauss_s, "lanczos_steps": int(it_g[0]),
+                              "omega_max_rel_diff_vs_ones_start": float(np.max(np.abs(omega_g - omega) / omega))},
            "max_relative_residual": {"nfsep": float((r_nf / nx).max()), "original": float((r_org / nf).max())}}
     if not args.no_cpu:
         log("config1: CPU baseline (the oracle's Lanczos on the host cores)")
