@@ -441,7 +441,7 @@ void material_shape(const fame_plan* P, int cnb, int& TX, int& TY, int& TZ, int&
 
 // number of per-work-group partial sums the material pass leaves per item (dpart below)
 int material_parts(const fame_plan* P, int cnb) {
-  if (P->pv) return (int)grid1d(P->N);
+  if (P->pv && !P->fast) return (int)grid1d(P->N);
   int TX, TY, TZ, ntile, nzc;
   material_shape(P, cnb, TX, TY, TZ, ntile, nzc);
   return ntile * nzc;
@@ -451,14 +451,24 @@ int material_parts(const fame_plan* P, int cnb) {
 // stride pstride
 int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs, double2* dpart = nullptr,
              int pstride = 0) {
-  if (P->pv) return pv_material(P, in, ibs, out, obs, dpart, pstride);
+  if (P->pv && !P->fast) return pv_material(P, in, ibs, out, obs, dpart, pstride);
   if (P->fast) {
     int TX, TY, TZ, ntile, nzc;
     material_shape(P, P->cnb, TX, TY, TZ, ntile, nzc);
     if (dpart && ntile * nzc > pstride) return set_err(FAME_ERR_ARG, "material: partial-sum stride too small");
-    hipLaunchKernelGGL(k_material_z, dim3(ntile, nzc, P->cnb), dim3(kMzThreads), material_z_lds(TX, TY), P->stream,
-                       P->geo, in, ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, wom(P), TX, TY, TZ, dpart,
-                       pstride);
+    const dim3 grid(ntile, nzc, P->cnb);
+    if (P->pv) {
+      PvCoef c;
+      for (int i = 0; i < 9; ++i) c.G[i] = P->pv_G[i];
+      c.mu = P->pv_mu;
+      hipLaunchKernelGGL(k_material_z<true>, grid, dim3(kMzThreads), material_z_lds(TX, TY, true), P->stream, P->geo,
+                         in, ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, P->d_eps, P->d_gam, c, wom(P), TX, TY,
+                         TZ, dpart, pstride);
+    } else {
+      hipLaunchKernelGGL(k_material_z<false>, grid, dim3(kMzThreads), material_z_lds(TX, TY), P->stream, P->geo, in,
+                         ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, nullptr, nullptr, PvCoef{}, wom(P), TX, TY,
+                         TZ, dpart, pstride);
+    }
     CHECK_LAUNCH();
     return FAME_OK;
   }
@@ -899,7 +909,9 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<1, ZM_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_MIX_OUT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
   PLAN_HIP(hipFuncSetAttribute((const void*)k_zpass<-1, ZM_PLAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-  PLAN_HIP(hipFuncSetAttribute((const void*)k_material_z, hipFuncAttributeMaxDynamicSharedMemorySize,
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_material_z<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)material_z_lds_max()));
+  PLAN_HIP(hipFuncSetAttribute((const void*)k_material_z<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)material_z_lds_max()));
 #undef PLAN_HIP
   {
@@ -1052,7 +1064,7 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   }
   P->chunk = bestc;
   // material stencil tile and z-slab: the heuristic's best few shapes, each with three slab lengths
-  if (P->fast && !P->pv) {
+  if (P->fast) {
     struct Cand { double score; int tx, ty; };
     std::vector<Cand> tiles;
     for (int tx = 4; tx <= std::min(P->n1, 30); ++tx)

@@ -50,16 +50,26 @@ def pv_setup(pkg, dims, gamma=0.2, seed=5):
 THETAS = np.array([[0.4, -1.3, 2.2], [1.5, 0.1, -0.7]])
 
 
-@pytest.mark.parametrize("dims", [(6, 5, 4), (8, 8, 8)])
-def test_vertex_material_fncB_matches_oracle(pkg, fo, torch_mod, dims):
-    say(f"per-vertex fncB dims={dims}")
+# a skewed lattice (non-diagonal metric G = A^T A) and mu != 1 exercise every slot of the
+# closed-form per-vertex product; "generic" runs the two-kernel form instead of the z-march
+SKEW = np.array([[1.0, 0.5, 0.2], [0.0, 0.9, 0.3], [0.1, 0.0, 1.1]])
+
+
+@pytest.mark.parametrize("dims,lattice,mu,generic", [
+    ((6, 5, 4), "eye", 1.0, False), ((8, 8, 8), "eye", 1.0, False),
+    ((8, 8, 8), "skew", 1.7, False), ((6, 5, 4), "skew", 1.7, False), ((8, 8, 8), "skew", 1.7, True)])
+def test_vertex_material_fncB_matches_oracle(pkg, fo, torch_mod, dims, lattice, mu, generic):
+    say(f"per-vertex fncB dims={dims} lattice={lattice} mu={mu} generic={generic}")
+    A = np.eye(3) if lattice == "eye" else SKEW
     ids, eps, gam = pv_setup(pkg, dims)
     table = pkg.structures.bi_isotropic_table(13.0, 0.2, np.eye(3))
     plan = pkg.Plan(dims, THETAS, table, ids)
-    plan.set_vertex_material(eps, gam)
+    plan.set_vertex_material(eps, gam, mu, A)
+    if generic:
+        plan.set_generic(True)
     x = crand(2, 4 * plan.N)
     y = plan.fncB(torch_mod.as_tensor(x)).cpu().numpy()
-    Nv, Mv, Jv = fo.vertex_material(eps, gam, 1.0, np.eye(3))
+    Nv, Mv, Jv = fo.vertex_material(eps, gam, mu, A)
     for b in range(2):
         P = fo.Problem(dims, THETAS[b], Nv, Mv, Jv)
         assert rel(y[b], P.fncB(x[b])) < 1e-12
