@@ -537,8 +537,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
-                                   "(k-points shard across GPUs with no collectives, see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark measures one GPU only; --gpus must be 1 "
+                                   "(k-points are independent eigenproblems: a band path would shard "
+                                   "as K/G k-points per GPU with no collectives)"}))
         sys.exit(2)
 
     import torch
