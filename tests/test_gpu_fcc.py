@@ -120,6 +120,46 @@ def test_nfgep_and_eigenvector(pkg, torch_mod):
     plan.close()
 
 
+@pytest.mark.parametrize("n", [8, 12])
+def test_nfgep_fused_matches_oracle(pkg, torch_mod, n):
+    """Cubic grids of a specialised length run the fused NFGEP kernels (Lambda products and
+    the T z stage in one launch, T xy + mid + T^H xy in one, T^H z + adjoint products in one)."""
+    say(f"FCC fused NFGEP / Compute_eigenvector n={n}")
+    plan, tab = make(pkg, (n, n, n), K)
+    svd = fc.svd_curl(tab)
+    N = plan.n
+    x = crand(32 * N)
+    xi, zeta, phi = crand(24 * N), crand(24 * N), crand(24 * N) + 3
+    t = torch_mod
+    y = plan.mtx_prod_vec_A_NFGEP(t.as_tensor(x), t.as_tensor(xi), t.as_tensor(zeta), t.as_tensor(phi))
+    assert rel(y.cpu().numpy(), fc.nfgep_A_matvec(x, tab, svd, xi, zeta, phi)) < RTOL
+    ev = plan.Compute_eigenvector(t.as_tensor(x), t.as_tensor(xi), t.as_tensor(zeta), t.as_tensor(phi))
+    assert rel(ev.cpu().numpy(), fc.compute_eigenvector(x, tab, svd, xi, zeta, phi)) < RTOL
+    plan.close()
+
+
+@pytest.mark.parametrize("n", [n for n in FAST if n <= 64])
+def test_nfgep_fused_equals_generic(pkg, torch_mod, n):
+    """The fused NFGEP kernels (every specialised length up to 64) give the runtime-radix
+    path's matvec and eigenvector map to rounding."""
+    say(f"FCC fused NFGEP == generic n={n}")
+    plan, _ = make(pkg, (n, n, n), K)
+    N = plan.n
+    t = torch_mod
+    dev = dict(device="cuda")
+    x = t.as_tensor(crand(32 * N), **dev)
+    xi, zeta = t.as_tensor(crand(24 * N), **dev), t.as_tensor(crand(24 * N), **dev)
+    phi = t.as_tensor(crand(24 * N) + 3, **dev)
+    yf = plan.mtx_prod_vec_A_NFGEP(x, xi, zeta, phi)
+    ef = plan.Compute_eigenvector(x, xi, zeta, phi)
+    plan.set_generic(True)
+    yg = plan.mtx_prod_vec_A_NFGEP(x, xi, zeta, phi)
+    eg = plan.Compute_eigenvector(x, xi, zeta, phi)
+    assert float((yf - yg).norm() / yg.norm()) < 1e-13
+    assert float((ef - eg).norm() / eg.norm()) < 1e-13
+    plan.close()
+
+
 def test_material_operators(pkg, torch_mod):
     say("FCC mtx_Eps_Mu_Zeta_Xi / GEP B")
     dims = (4, 6, 6)
