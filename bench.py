@@ -271,6 +271,82 @@ def run_config1(pkg, torch, args):
     print(json.dumps(out), flush=True)
 
 
+def run_sweep(pkg, torch, args):
+    """fncB matvecs/s at n = 24, 48, 60, 72 (SC woodpile, gamma = 0.2): batched over the 40-k
+    band path (autotuned plan) and plain (one k-point per call), algorithmic GB/s on the 512 B
+    per point model and the fraction of HBM peak; beside it the oracle's CPU fncB at n = 24,
+    48, 60 (median of up to 20 calls after 3 warm-ups, capped at about --cpu-seconds per n)."""
+    import ctypes
+
+    S = pkg.structures
+    ks = S.sc_band_path()
+    nk = args.nk
+    idx = np.linspace(0, len(ks) - 1, nk).round().astype(int) if nk != len(ks) else np.arange(len(ks))
+    thetas = np.stack([S.theta_of_k(ks[i], S.SC_LATTICE) for i in idx])
+    table = S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE)
+    rows = []
+    for n in (24, 48, 60, 72):
+        dims, N = (n, n, n), n ** 3
+        ids = S.woodpile_sc_ids(dims)
+        pb = pkg.Plan(dims, thetas, table, ids)
+        p1 = pkg.Plan(dims, thetas[:1], table, ids)
+        chunk, _ = pb.autotune()
+        g = torch.Generator(device="cuda")
+        g.manual_seed(n)
+        xb = torch.randn(nk, 4 * N, dtype=torch.complex128, device="cuda", generator=g)
+        yb = torch.empty_like(xb)
+        lib = pb.lib
+
+        def call(pl, x, y):
+            if lib.fame_fncB(pl.h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr())) != 0:
+                raise RuntimeError(lib.fame_last_error().decode())
+
+        ms_b = _time_op(torch, lambda: call(pb, xb, yb), args.steps, warm=args.warmup)
+        ms_p = _time_op(torch, lambda: call(p1, xb[0], yb[0]), args.steps * 4, warm=args.warmup)
+        rb, rp = nk / (ms_b * 1e-3), 1.0 / (ms_p * 1e-3)
+        row = {"n": n, "batched_matvecs_per_s": rb, "plain_matvecs_per_s": rp, "batched_over_plain": rb / rp,
+               "chunk": chunk, "algo_GBs_batched": rb * FNCB_ALGO_BYTES_PER_POINT * N / 1e9,
+               "hbm_frac_batched": rb * FNCB_ALGO_BYTES_PER_POINT * N / 1e9 / HBM_PEAK_GBS,
+               "hbm_frac_plain": rp * FNCB_ALGO_BYTES_PER_POINT * N / 1e9 / HBM_PEAK_GBS}
+        log(f"sweep: n={n} batched {rb:.0f}/s (chunk {chunk}, {row['hbm_frac_batched']:.2f} of HBM peak), "
+            f"plain {rp:.0f}/s")
+        pb.close()
+        p1.close()
+        del xb, yb
+        if not args.no_cpu and n <= 60:
+            from oracle import fame_oracle as fo
+
+            T = table[ids]
+            P = fo.Problem(dims, thetas[0], T[:, 0], T[:, 1], T[:, 2])
+            x = np.random.default_rng(1234).standard_normal(4 * N) + 0j
+            ts = []
+            for i in range(23):
+                t0 = time.perf_counter()
+                P.fncB(x)
+                dt = time.perf_counter() - t0
+                if i >= 3:
+                    ts.append(dt)
+                if i >= 3 and sum(ts) > args.cpu_seconds:
+                    break
+            row["cpu_matvecs_per_s"] = 1.0 / float(np.median(ts))
+            row["cpu_calls_timed"] = len(ts)
+            log(f"sweep: n={n} CPU oracle fncB {row['cpu_matvecs_per_s']:.2f}/s ({len(ts)} calls)")
+        rows.append(row)
+        print(f"sweep: n={n} done", file=sys.stderr, flush=True)
+    r60 = next(r for r in rows if r["n"] == 60)
+    out = {"metric": "fncB matvecs/s by grid size n = 24/48/60/72 (batched 40 k vs plain one k), %HBM peak",
+           "value": r60["batched_matvecs_per_s"], "unit": "matvecs/s (n=60 batched)", "higher_is_better": True,
+           "n_gpus": 1, "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": "SC woodpile eps=13 gamma=0.2, 40 k on Gamma-X-M-R-Gamma, fncB on complex "
+                                  "Gaussian vectors; CPU: oracle fncB (scipy.fft + CSR SpMV), one k"},
+           "rows": rows, "cpu_cores": None}
+    if not args.no_cpu:
+        from oracle import fame_oracle as fo
+
+        out["cpu_cores"] = fo.FFT_WORKERS
+    print(json.dumps(out), flush=True)
+
+
 def run_config3(pkg, torch, args):
     """64 k-points over three FCC sphere crystals (r = 0.12a, eps = 13, gamma ~ U[0, 3.4] from
     default_rng(3)) on grids n = 48, 60, 72; 12 eigenpairs per k.  Batched: one batched plan per
@@ -532,6 +608,8 @@ def main():
                     help="with --chunk: FFT launch shapes (xy 0/1, z 0/1/2) instead of the defaults")
     ap.add_argument("--mtile", type=int, nargs=3, metavar=("TX", "TY", "TZ"),
                     help="with --chunk: material-stencil tile instead of the heuristic")
+    ap.add_argument("--sweep", action="store_true",
+                    help="fncB matvec rates at n = 24, 48, 60, 72 (batched and plain) with the CPU oracle's")
     ap.add_argument("--chunk", type=int, default=-1,
                     help="k-points per fncB chunk (-1: autotune on the plan, 0: whole batch)")
     args = ap.parse_args()
@@ -546,6 +624,10 @@ def main():
     import __graft_entry__ as ge
 
     pkg = ge.load_package()
+    if args.sweep:
+        torch.cuda.set_device(0)
+        run_sweep(pkg, torch, args)
+        return
     if args.config != 2:
         torch.cuda.set_device(0)
         if args.config == 5 and args.n == 60:

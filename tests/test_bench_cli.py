@@ -21,7 +21,7 @@ def test_help_lists_configs():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0
-    for opt in ("--config", "--chunk", "--variant", "--mtile", "--solve", "--concurrent", "--no-cpu"):
+    for opt in ("--config", "--chunk", "--variant", "--mtile", "--solve", "--concurrent", "--no-cpu", "--sweep"):
         assert opt in r.stdout
 
 
