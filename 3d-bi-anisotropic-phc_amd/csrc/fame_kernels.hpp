@@ -433,29 +433,6 @@ __global__ void __launch_bounds__(256) k_reduce_part(const double2* __restrict__
   if (threadIdx.x == 0) out[b] = acc;
 }
 
-// y[b] = a[b] * x[b] + y[b]   (a on device, complex)
-__global__ void k_axpy(const double2* __restrict__ a, const double2* __restrict__ x, long long xb, double2* __restrict__ y,
-                       long long yb, long long len, double sgn) {
-  const int b = blockIdx.y;
-  const double2 s = cscale(a[b], sgn);
-  const double2* xp = x + b * xb;
-  double2* yp = y + b * yb;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)
-    yp[i] = cfma(yp[i], s, xp[i]);
-}
-
-// y[b] = x[b] * s[b]   (real per-batch scale on device; s = 1/sqrt(part) when inv_sqrt)
-__global__ void k_scale(const double2* __restrict__ x, long long xb, double2* __restrict__ y, long long yb, long long len,
-                        const double2* __restrict__ s, int inv_sqrt) {
-  const int b = blockIdx.y;
-  double f = s[b].x;
-  if (inv_sqrt) f = 1.0 / sqrt(fabs(f));
-  const double2* xp = x + b * xb;
-  double2* yp = y + b * yb;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x)
-    yp[i] = cscale(xp[i], f);
-}
-
 // ---------------------------------------------------------------------------
 // Conjugate gradients on B (fncinvB), all batch items in lock step.
 // Scalars per batch item live on the device:
