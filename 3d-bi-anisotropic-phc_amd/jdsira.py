@@ -73,6 +73,19 @@ def _norm(plan, x):
     return float(np.sqrt(abs(_dot(plan, x, x))))
 
 
+def _norm_dev(plan, x):
+    """||x|| as a 0-dim device tensor: for normalisations that need no host decision, so the
+    stream is not drained for a round trip."""
+    return plan.block_dot(x.reshape(1, -1), x)[0].abs().sqrt()
+
+
+def _proj_cols(plan, V, AV, BV):
+    """V^H AV and V^H BV, fetched to the host in one transfer."""
+    t = plan.torch
+    both = t.stack([plan.block_dot(V, AV), plan.block_dot(V, BV)]).cpu().numpy()
+    return both[0], both[1]
+
+
 def choose_ritz_pair(iteno, ew, ew_order, target, VRR, vec_r_tol, choose_type, rppd, target_org, choose_type_org,
                      zflag_ew, flag_no_ew):
     """Select Ritz pairs (Choose_Ritz_Pair_Unsymm4.m:1-412); returns (ewno, ritz_ew, ritz_ev)."""
@@ -251,12 +264,18 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
     iteno = proj_dim
     rsdl = 0.0
     while EPIteNo <= 6000 and stop_flag == 0:
-        for ii in range(proj_dim):  # Rayleigh quotients (:91-103)
-            AV = A(V[ii])
-            Mm[ii:proj_dim, ii] = plan.block_dot(V[ii:proj_dim], AV).cpu().numpy()
+        cols = []  # Rayleigh quotients (:91-103); columns stay on the device until all are done
+        for ii in range(proj_dim):
+            cols.append(plan.block_dot(V[ii:proj_dim], A(V[ii])))
+            cols.append(plan.block_dot(V[ii:proj_dim], B(V[ii])))
+        host = t.cat(cols).cpu().numpy() if cols else np.zeros(0, complex)
+        off = 0
+        for ii in range(proj_dim):
+            w = proj_dim - ii
+            Mm[ii:proj_dim, ii] = host[off:off + w]
+            Lm[ii:proj_dim, ii] = host[off + w:off + 2 * w]
+            off += 2 * w
             Mm[ii, ii + 1:proj_dim] = np.conj(Mm[ii + 1:proj_dim, ii])
-            BV = B(V[ii])
-            Lm[ii:proj_dim, ii] = plan.block_dot(V[ii:proj_dim], BV).cpu().numpy()
             Lm[ii, ii + 1:proj_dim] = np.conj(Lm[ii + 1:proj_dim, ii])
         iteno = proj_dim
         while iteno <= no_restart and stop_flag == 0:
@@ -267,7 +286,7 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
                                                       choose_type, rppd, target, choose_type_org, st.zflag_ew,
                                                       st.flag_no_ew)
             ev_crnt = plan.block_gemv(V[:iteno], t.as_tensor(ritz_ev[:iteno, 0]))
-            ev_crnt = ev_crnt / _norm(plan, ev_crnt)
+            ev_crnt = ev_crnt / _norm_dev(plan, ev_crnt)
             rsdl_vec = A(ev_crnt) - ritz_ew[0] * B(ev_crnt)
             rsdl = _norm(plan, rsdl_vec)
             st.log.append((ew_order, EPIteNo, complex(ritz_ew[0]), rsdl))
@@ -299,17 +318,14 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
                     sig = 0.0 if sigma_inner is None else float(sigma_inner)
                     tv, its, _ = plan.minres(rsdl_vec, sigma=sig, tol=linear_system_tol, maxit=pcg_iter_number)
                     st.total_inner_iter += int(its[0])
-                    nrm = _norm(plan, tv)
-                    if nrm < 1.0e-9:
-                        tv = tv / nrm
-                    tv = tv.contiguous()
+                    nrm = _norm_dev(plan, tv)  # a tiny correction is rescaled first (:318-320)
+                    tv = (tv * t.where(nrm < 1.0e-9, 1.0 / nrm, t.ones_like(nrm))).contiguous()
                     plan.mgs(V[:iteno], tv)  # orthogonalise against V, MGS in column order (:326-330)
-                    V[iteno] = tv / _norm(plan, tv)
-                    AV = A(V[iteno])  # update V^H A V and V^H B V (:335-341)
-                    Mm[:iteno + 1, iteno] = plan.block_dot(V[:iteno + 1], AV).cpu().numpy()
+                    V[iteno] = tv / _norm_dev(plan, tv)
+                    # update V^H A V and V^H B V (:335-341)
+                    Mm[:iteno + 1, iteno], Lm[:iteno + 1, iteno] = _proj_cols(plan, V[:iteno + 1], A(V[iteno]),
+                                                                              B(V[iteno]))
                     Mm[iteno, :iteno] = np.conj(Mm[:iteno, iteno])
-                    BV = B(V[iteno])
-                    Lm[:iteno + 1, iteno] = plan.block_dot(V[:iteno + 1], BV).cpu().numpy()
                     Lm[iteno, :iteno] = np.conj(Lm[:iteno, iteno])
             iteno += 1
             EPIteNo += 1
@@ -339,7 +355,7 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
             tmp_rsdl[i] = _norm(plan, rv)
     st.flag_no_ew += 1
     k = st.dim_eigsp
-    u = ev_crnt / _norm(plan, ev_crnt)  # lock the converged pair (:425-450)
+    u = ev_crnt / _norm_dev(plan, ev_crnt)  # lock the converged pair (:425-450)
     import time as _time
     st.locked.append((ew_order, EPIteNo, _time.perf_counter() - st.t0, complex(ritz_ew[0]), rsdl))
     V[k] = u
