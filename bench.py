@@ -208,6 +208,47 @@ def cpu_baseline_solve(n, theta, table, ids, nev, tol, p):
     return time.perf_counter() - t0, np.sort(1.0 / np.real(ew)), int(steps), float(np.mean(its)), fo.FFT_WORKERS
 
 
+class _CpuCap(Exception):
+    pass
+
+
+def cpu_solve_estimate(n, theta, table, ids, nev, p, cap_s, gpu_steps):
+    """Time-capped CPU baseline of one k-point solve at n: the oracle's Lanczos (as in
+    cpu_baseline_solve, v0 = ones) stopped at about ``cap_s`` seconds.  Estimate = setup +
+    (seconds per Lanczos step so far) x the step count the GPU solve of the same k needed."""
+    from oracle import fame_oracle as fo
+
+    dims = (n, n, n)
+    t0 = time.perf_counter()
+    T = table[ids]
+    P = fo.Problem(dims, theta, T[:, 0], T[:, 1], T[:, 2])
+    t_setup = time.perf_counter() - t0
+    its = []
+
+    def binv(x):
+        if time.perf_counter() - t0 > cap_s:
+            raise _CpuCap()
+        y, _, _, it = P.fncinvB(x, 1e-12, 10000)
+        its.append(it)
+        log(f"bench: CPU solve estimate: {len(its)} Lanczos steps, {time.perf_counter() - t0:.0f} s")
+        return y
+
+    finished = True
+    try:
+        fo.lanczos(P.fncA, P.fncB, binv, 4 * P.N, nev, 1e-10, 10000, p, np.ones(4 * P.N, complex))
+    except _CpuCap:
+        finished = False
+    elapsed = time.perf_counter() - t0
+    per_step = (elapsed - t_setup) / max(len(its), 1)
+    value = elapsed if finished else t_setup + per_step * gpu_steps
+    return {"value": value, "unit": "s/k-point", "cores": fo.FFT_WORKERS, "kind": "port",
+            "sample": (f"oracle Lanczos (CPU restatement of Lanczos.m, pcg, fncB) at n={n} on one k-point: "
+                       + ("ran to convergence" if finished else
+                          f"stopped after {len(its)} Lanczos steps ({elapsed:.1f} s, {np.mean(its):.1f} CG "
+                          f"iterations per step); extrapolated as setup {t_setup:.1f} s + {per_step:.2f} s per "
+                          f"step x the {gpu_steps} steps the GPU solve of this k-point needed"))}
+
+
 def run_config1(pkg, torch, args):
     """The reference's CPU-runnable case: SC woodpile n = 24, eps = 13, gamma = 0.2, one k-point
     pi (0.5, 0.3, 0.1), the 10 smallest omega by Lanczos (p = 20, tol 1e-10, CG tol 1e-12,
@@ -844,6 +885,16 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+        if args.solve != "none":
+            # one k-point of the plain sample with the median step count, time-capped
+            jm = int(np.argsort(steps)[len(steps) // 2])
+            log(f"bench: CPU solve estimate on k#{sample[jm]} ({steps[jm]} GPU Lanczos steps)")
+            est = cpu_solve_estimate(n, thetas[sample[jm]], table, ids, args.nev, 2 * args.nev,
+                                     args.cpu_seconds + 10.0, steps[jm])
+            est["k_index"] = int(sample[jm])
+            est["gpu_plain_s"] = times[jm]
+            out["cpu_baseline_solve"] = est
+            log(f"bench: CPU solve estimate {est['value']:.0f} s/k-point against {times[jm]:.2f} s plain on the GPU")
     pb.close()
     p1.close()
     print(json.dumps(out), flush=True)
