@@ -138,11 +138,11 @@ def test_nfgep_fused_matches_oracle(pkg, torch_mod, n):
     plan.close()
 
 
-@pytest.mark.parametrize("n", [n for n in FAST if n <= 64])
-def test_nfgep_fused_equals_generic(pkg, torch_mod, n):
-    """The fused NFGEP kernels (every specialised length up to 64) give the runtime-radix
-    path's matvec and eigenvector map to rounding."""
-    say(f"FCC fused NFGEP == generic n={n}")
+@pytest.mark.parametrize("n", FAST)
+def test_nfgep_specialised_equals_generic(pkg, torch_mod, n):
+    """The specialised NFGEP path (fused kernels up to length 64, separate specialised passes
+    above) gives the runtime-radix path's matvec and eigenvector map to rounding."""
+    say(f"FCC specialised NFGEP == generic n={n}")
     plan, _ = make(pkg, (n, n, n), K)
     N = plan.n
     t = torch_mod
