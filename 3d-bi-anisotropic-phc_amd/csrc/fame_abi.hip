@@ -599,20 +599,21 @@ int pack_rows(fame_plan* P, const double2* src, double2* dst, long long len, int
   return FAME_OK;
 }
 
-int do_fncA(fame_plan* P, const double2* x, double2* y) {
+// act (optional, [vnb]): items with act[b] < 0 are skipped
+int do_fncA(fame_plan* P, const double2* x, double2* y, const int* act = nullptr) {
   hipLaunchKernelGGL(k_fncA, dim3(grid1d(P->N), P->vnb), dim3(kBlk), 0, P->stream, P->geo, x, y, P->v_lam,
-                     P->tab_len);
+                     P->tab_len, act);
   CHECK_LAUNCH();
   return FAME_OK;
 }
 
-// out[b] = x[b]^H y[b] (device, P->d_red or given)
+// out[b] = x[b]^H y[b] (device, P->d_red or given); items with act[b] < 0 (act optional) get 0
 int do_dot(fame_plan* P, const double2* x, long long xbs, const double2* y, long long ybs, long long len, int nb,
-           double2* out) {
+           double2* out, const int* act = nullptr) {
   const int nblk = std::min<int>(P->nblk_red, (int)grid1d(len));
-  hipLaunchKernelGGL(k_dot_part, dim3(nblk, nb), dim3(kBlk), 0, P->stream, x, xbs, y, ybs, len, P->d_part);
+  hipLaunchKernelGGL(k_dot_part, dim3(nblk, nb), dim3(kBlk), 0, P->stream, x, xbs, y, ybs, len, P->d_part, act);
   CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_reduce_part, dim3(nb), dim3(kBlk), 0, P->stream, P->d_part, nblk, out);
+  hipLaunchKernelGGL(k_reduce_part, dim3(nb), dim3(kBlk), 0, P->stream, P->d_part, nblk, out, act);
   CHECK_LAUNCH();
   return FAME_OK;
 }

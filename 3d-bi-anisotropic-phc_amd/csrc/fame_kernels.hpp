@@ -359,9 +359,10 @@ __global__ void __launch_bounds__(256) k_material(Geo g, const double2* __restri
 // fncA (fun_Lanczos/fncA.m:3-4): y = [ i inv_r .* x_b ; -i inv_r .* x_a ],
 // inv_r = 1/Sigma_r, Sigma_r = [sqrt(Lq); sqrt(Lq)] recomputed from lambda.
 __global__ void k_fncA(Geo g, const double2* __restrict__ x, double2* __restrict__ y, const double2* __restrict__ lam,
-                       int lam_bstride) {
+                       int lam_bstride, const int* __restrict__ act) {
   const long long N = g.N;
   const int b = blockIdx.y;
+  if (act && act[b] < 0) return;  // item b skipped (act == nullptr: every item)
   const double2* __restrict__ lb = lam + (long long)b * lam_bstride;
   const double2* __restrict__ xb = x + (long long)b * 4 * N;
   double2* __restrict__ yb = y + (long long)b * 4 * N;
@@ -409,8 +410,9 @@ __device__ __forceinline__ double2 block_sum(double2 v) {
 // part[b*gridDim.x + blk] = sum over this block's range of conj(x) .* y
 __global__ void __launch_bounds__(256) k_dot_part(const double2* __restrict__ x, long long xb,
                                                   const double2* __restrict__ y, long long yb, long long len,
-                                                  double2* __restrict__ part) {
+                                                  double2* __restrict__ part, const int* __restrict__ act) {
   const int b = blockIdx.y;
+  if (act && act[b] < 0) return;  // uniform over the work-group
   const double2* xp = x + b * xb;
   const double2* yp = y + b * yb;
   double2 acc = make_double2(0, 0);
@@ -423,10 +425,14 @@ __global__ void __launch_bounds__(256) k_dot_part(const double2* __restrict__ x,
   if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = acc;
 }
 
-// out[b] = sum of part[b][0:nblk]  (one block per batch item)
+// out[b] = sum of part[b][0:nblk]  (one block per batch item); items with act[b] < 0 get 0
 __global__ void __launch_bounds__(256) k_reduce_part(const double2* __restrict__ part, int nblk,
-                                                     double2* __restrict__ out) {
+                                                     double2* __restrict__ out, const int* __restrict__ act) {
   const int b = blockIdx.x;
+  if (act && act[b] < 0) {
+    if (threadIdx.x == 0) out[b] = make_double2(0, 0);
+    return;
+  }
   double2 acc = make_double2(0, 0);
   for (int i = threadIdx.x; i < nblk; i += blockDim.x) acc = cadd(acc, part[b * nblk + i]);
   acc = block_sum(acc);
