@@ -245,23 +245,23 @@ size_t xyfast_lds(int n) { return (size_t)(n * (n % 2 == 0 ? n + 1 : n) + n) * s
 
 template <int N, int A, int B, int NT>
 int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs,
-                   int nf, double sc) {
+                   int nf, double sc, const CgState* cgst) {
   const dim3 grid(P->n3, nf, P->cnb);
   const size_t lds = xyfast_lds(N);
   if (dir > 0) {
     if (twist == 1)
       hipLaunchKernelGGL((k_xyfast<N, A, B, 1, 1, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         wtws(P), P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc, cgst);
     else
       hipLaunchKernelGGL((k_xyfast<N, A, B, 1, 0, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         wtws(P), P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc, cgst);
   } else {
     if (twist == 2)
       hipLaunchKernelGGL((k_xyfast<N, A, B, -1, 2, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         wtws(P), P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc, cgst);
     else
       hipLaunchKernelGGL((k_xyfast<N, A, B, -1, 0, NT>), grid, dim3(NT), lds, P->stream, P->geo, in, ibs, out, obs,
-                         wtws(P), P->tab_len, P->d_tw, sc);
+                         wtws(P), P->tab_len, P->d_tw, sc, cgst);
   }
   CHECK_LAUNCH();
   return FAME_OK;
@@ -374,13 +374,15 @@ int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, dou
   return FAME_OK;
 }
 
-int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs, int nf) {
+// cgst (fast kernels only, optional): inside a fused CG, items whose state is inactive are skipped
+int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, double2* out, long long obs, int nf,
+           const CgState* cgst = nullptr) {
   const double sc = 1.0 / std::sqrt((double)P->n1 * (double)P->n2);
   if (P->fast && use_fast_xy(P)) {
 #define X(N, A, B)                                                                                    \
     if (P->n1 == N) {                                                                                   \
-      if (P->xyv == 1) return xy_fast_launch<N, A, B, 512>(P, dir, twist, in, ibs, out, obs, nf, sc);  \
-      return xy_fast_launch<N, A, B, xy_threads<N>()>(P, dir, twist, in, ibs, out, obs, nf, sc);       \
+      if (P->xyv == 1) return xy_fast_launch<N, A, B, 512>(P, dir, twist, in, ibs, out, obs, nf, sc, cgst); \
+      return xy_fast_launch<N, A, B, xy_threads<N>()>(P, dir, twist, in, ibs, out, obs, nf, sc, cgst);      \
     }
     FAME_FAST_SIZES(X)
 #undef X
@@ -449,8 +451,9 @@ int material_parts(const fame_plan* P, int cnb) {
 
 // dpart (specialised and per-vertex stencils): per-work-group sums of conj(in) . out, item
 // stride pstride
+// cgst (specialised stencil only, optional): inside a fused CG, items whose state is inactive are skipped
 int material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs, double2* dpart = nullptr,
-             int pstride = 0) {
+             int pstride = 0, const CgState* cgst = nullptr) {
   if (P->pv && !P->fast) return pv_material(P, in, ibs, out, obs, dpart, pstride);
   if (P->fast) {
     int TX, TY, TZ, ntile, nzc;
@@ -463,11 +466,11 @@ int material(fame_plan* P, const double2* in, long long ibs, double2* out, long 
       c.mu = P->pv_mu;
       hipLaunchKernelGGL(k_material_z<true>, grid, dim3(kMzThreads), material_z_lds(TX, TY, true), P->stream, P->geo,
                          in, ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, P->d_eps, P->d_gam, c, wom(P), TX, TY,
-                         TZ, dpart, pstride);
+                         TZ, dpart, pstride, cgst);
     } else {
       hipLaunchKernelGGL(k_material_z<false>, grid, dim3(kMzThreads), material_z_lds(TX, TY), P->stream, P->geo, in,
                          ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, nullptr, nullptr, PvCoef{}, wom(P), TX, TY,
-                         TZ, dpart, pstride);
+                         TZ, dpart, pstride, cgst);
     }
     CHECK_LAUNCH();
     return FAME_OK;
@@ -516,16 +519,18 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   // MIX_OUT pass updates r in place (y is r then)
   RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0, pcf));
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
-  RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
+  // converged items of a fused CG are skipped by every pass (their state is frozen)
+  const CgState* cgst = pcf ? wcf.st : nullptr;
+  RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
-  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N, pcf ? wcf.part : nullptr, pcf ? wcf.nparts : 0));
+  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N, pcf ? wcf.part : nullptr, pcf ? wcf.nparts : 0, cgst));
   if (pcf) {
     hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, material_parts(P, nbc), wcf.nparts,
                        wcf.st);
     CHECK_LAUNCH();
   }
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
-  RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6));
+  RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
   RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, pcf));
   if (pr) {

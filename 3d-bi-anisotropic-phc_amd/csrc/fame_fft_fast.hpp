@@ -72,7 +72,8 @@ template <int N, int A, int B, int DIR, int TWIST, int NT>
 __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict__ in, long long in_bstride,
                                                double2* __restrict__ out, long long out_bstride,
                                                const double2* __restrict__ tws, int tws_bstride,
-                                               const double2* __restrict__ twg, double scale) {
+                                               const double2* __restrict__ twg, double scale,
+                                               const CgState* __restrict__ cgst) {
   static_assert(A * B == N, "k_xyfast: N = A*B");
   // odd row pitch: lanes that walk down a column hit distinct LDS banks
   constexpr int PT = (N % 2 == 0) ? N + 1 : N;
@@ -80,6 +81,7 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
   double2* pl = sm;
   double2* tw = sm + N * PT;
   const int i3 = blockIdx.x, f = blockIdx.y, b = blockIdx.z;
+  if (cgst && cgst[b].active == 0.0) return;  // inside a fused CG: item b has converged
   const long long base = (long long)f * g.N + (long long)i3 * (N * N);
   const double2* __restrict__ src = in + b * in_bstride + base;
   double2* __restrict__ dst = out + b * out_bstride + base;
@@ -200,6 +202,8 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
   const int slot = blockIdx.y;
   const int grp = grp_base + slot;
   const int b = blockIdx.z;
+  // a converged CG item is left as it is: its last deferred x update is applied by k_cg_xfin
+  if (CG && cf.st[b].active == 0.0) return;
   const double2* __restrict__ lb = lam + (long long)b * lam_bstride;
   const double2* __restrict__ twn = twg + g.ax[2].tw_off;
   for (int t = threadIdx.x; t < N3; t += NT) tw[t] = twn[t];

@@ -547,7 +547,9 @@ __global__ void __launch_bounds__(256) k_cg_alpha2(const double2* __restrict__ p
   acc = block_sum(acc);
   if (threadIdx.x == 0) {
     CgState s = st[b];
-    s.alpha = (s.active != 0.0) ? s.rho / acc.x : 0.0;
+    // a converged item keeps its last alpha: the fused passes skip it from then on, and
+    // k_cg_xfin applies its deferred update x += alpha p once at the end
+    if (s.active != 0.0) s.alpha = s.rho / acc.x;
     st[b] = s;
   }
 }

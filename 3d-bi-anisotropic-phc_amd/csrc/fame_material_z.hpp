@@ -73,7 +73,8 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
                                                            const double* __restrict__ veps,
                                                            const double* __restrict__ vgam, PvCoef pc,
                                                            const double2* __restrict__ omega, int TX, int TY, int TZ,
-                                                           double2* __restrict__ dpart, int pstride) {
+                                                           double2* __restrict__ dpart, int pstride,
+                                                           const CgState* __restrict__ cgst) {
   extern __shared__ double2 sm[];
   const int RFX = TX + 2, RFY = TY + 2, RF = RFX * RFY;
   double2* tab = sm;
@@ -98,6 +99,7 @@ __global__ void __launch_bounds__(kMzThreads) k_material_z(Geo g, const double2*
     zc = (int)((logical / gridDim.x) % gridDim.y);
     b = (int)(logical / (gridDim.x * gridDim.y));
   }
+  if (cgst && cgst[b].active == 0.0) return;  // inside a fused CG: item b has converged
   const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY;
   const int z0 = zc * TZ;
   const int z1 = min(z0 + TZ, n3);
