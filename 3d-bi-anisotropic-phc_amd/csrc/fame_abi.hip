@@ -109,6 +109,9 @@ struct fame_plan {
   double2* d_p = nullptr;
   double2* d_q = nullptr;
   CgState* d_cg = nullptr;
+  CgState* h_cg = nullptr;    // pinned host mirror of d_cg for the lagged convergence poll
+  int h_cg_cap = 0;
+  hipEvent_t ev_poll = nullptr;
   int tab_len = 0;            // n1+n2+n3
   bool fast = true;           // use the specialised FFT kernels where a size has them
   // batch window used by the launch helpers: items [cb0, cb0 + cnb) of the current view
@@ -745,14 +748,39 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
       }
     }
   } else {
-    // all items in lock step, chunk by chunk within an iteration
+    // all items in lock step, chunk by chunk within an iteration.  The convergence poll runs
+    // one batch of 4 iterations behind: after each batch the state goes to pinned host memory
+    // and is read once the next batch is queued, so the device never waits for the host.
+    // Iterations queued after the last item converged find every item inactive and return
+    // at once (converged items are skipped by every pass).
+    if (P->h_cg_cap < nb) {
+      if (P->h_cg) HIP_TRY(hipHostFree(P->h_cg));
+      P->h_cg = nullptr;
+      P->h_cg_cap = 0;
+      HIP_TRY(hipHostMalloc((void**)&P->h_cg, (size_t)nb * sizeof(CgState), hipHostMallocDefault));
+      P->h_cg_cap = nb;
+    }
+    if (!P->ev_poll) HIP_TRY(hipEventCreateWithFlags(&P->ev_poll, hipEventDisableTiming));
     const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
-    bool done = false;
+    bool done = false, pending = false;
     for (int it = 0; it < maxit && !done;) {
-      for (int b0 = 0; b0 < nb; b0 += c) RET_IF(step(b0, std::min(c, nb - b0)));
-      if (P->prof) P->prof_calls += 1;
-      ++it;
-      if (it % 4 == 0 || it == maxit) RET_IF(finished(0, nb, done));
+      const int batch = std::min(4, maxit - it);
+      for (int k = 0; k < batch; ++k) {
+        for (int b0 = 0; b0 < nb; b0 += c) RET_IF(step(b0, std::min(c, nb - b0)));
+        if (P->prof) P->prof_calls += 1;
+      }
+      it += batch;
+      if (pending) {  // the state after the previous batch
+        HIP_TRY(hipEventSynchronize(P->ev_poll));
+        done = true;
+        for (int i = 0; i < nb; ++i)
+          if (P->h_cg[i].active != 0.0) done = false;
+      }
+      if (!done) {
+        HIP_TRY(hipMemcpyAsync(P->h_cg, P->d_cg, (size_t)nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipEventRecord(P->ev_poll, P->stream));
+        pending = true;
+      }
     }
   }
   hipLaunchKernelGGL(k_cg_xfin, dim3(grid1d(L), nb), dim3(kBlk), 0, P->stream, x, P->d_p, L, P->d_cg);
@@ -948,6 +976,8 @@ int fame_plan_destroy(fame_plan* P) {
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
     if (e) (void)hipEventDestroy(e);
+  if (P->ev_poll) (void)hipEventDestroy(P->ev_poll);
+  if (P->h_cg) (void)hipHostFree(P->h_cg);
   if (P->own_stream) {
     (void)hipStreamSynchronize(P->own_stream);
     (void)hipStreamDestroy(P->own_stream);
