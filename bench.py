@@ -565,6 +565,11 @@ def run_config5(pkg, torch, args):
     pl.set_vertex_material(eps, gam)
     nev = 8
     lz = dict(tol=1e-12, maxiter=2000, p=2 * nev + 4)
+    # warm-up: module loading, kernel attributes and the first torch calls of both halves
+    wb = pkg.adjoint.BandGradient(pl)
+    wb.forward(2, tol=1e-6, maxiter=50, p=4)
+    pkg.adjoint.eigen_gradients(pl, wb.ev, wb.omega)
+    del wb
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bg = pkg.adjoint.BandGradient(pl)
