@@ -212,6 +212,15 @@ def test_fused_cg_matches_textbook_cg(pkg, torch_mod, n, chunk):
             P = oracle_problem(pkg_fo(), dims, thetas[b], table, ids)
             xb = x[b].cpu().numpy()
             assert np.linalg.norm(P.fncB(yf[b]) - xb) <= 2e-12 * np.linalg.norm(xb)
+    if n != 60:
+        # each item alone: converged items are skipped while the batch runs on, so an item's
+        # iteration count and solution do not depend on its neighbours
+        for b in (0, 2):
+            p1 = pkg.Plan(dims, thetas[b], table, ids)
+            y1, it1, _ = p1.fncinvB(x[b], tol=1e-12, maxit=3000)
+            assert int(it1[0]) == int(itf[b])
+            assert rel(y1.cpu().numpy(), yf[b]) < 1e-12
+            p1.close()
     plan.close()
 
 
