@@ -224,6 +224,29 @@ def test_fused_cg_matches_textbook_cg(pkg, torch_mod, n, chunk):
     plan.close()
 
 
+def test_cg_sequential_chunks_match_lock_step(pkg, torch_mod):
+    """Solving chunks of k-points to convergence one after another gives each item the
+    lock-step iterates (same iteration count, same solution), a zero right-hand side included."""
+    say("fused CG: sequential chunks == lock step")
+    dims = (12, 12, 12)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE)
+                       for k in ([0.5, 0.3, 0.1], [0.2, 0.0, 0.9], [0.7, 0.7, 0.1])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    x = torch_mod.as_tensor(crand(3, 4 * plan.N))
+    x[2] = 0
+    yl, itl, rl = plan.fncinvB(x, tol=1e-12, maxit=3000)
+    plan.set_cg_fused(True, seq_chunk=2)
+    ys, its, rs = plan.fncinvB(x, tol=1e-12, maxit=3000)
+    yl, ys = yl.cpu().numpy(), ys.cpu().numpy()
+    assert list(itl) == list(its)
+    assert not ys[2].any() and its[2] == 0
+    for b in (0, 1):
+        assert rel(ys[b], yl[b]) < 1e-12 and rs[b] <= 1e-12
+    plan.close()
+
+
 def test_cg_graph_replays_match_eager(pkg, torch_mod):
     """The fused CG replayed from a captured HIP graph gives the eagerly launched result on
     every call, with other device work and synchronisations between the calls."""
