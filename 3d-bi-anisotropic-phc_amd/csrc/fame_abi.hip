@@ -205,6 +205,53 @@ void fill_k_tables(const fame_plan* P, std::vector<double2>& lam, std::vector<do
   }
 }
 
+// Host check that the curl SVD is defined at every Fourier point of every item: Pi_0 and Pi_2
+// divide by sqrt(Lambda_q) and Pi_1 by sqrt(Lambda_q - |w . lambda|^2) (SVD_Matrix_curl.m:45-57),
+// and both vanish when lambda is parallel to w = (1,2,3)/sqrt(14), which includes lambda = 0 (the
+// Gamma point: theta_d = 0 mod 2 pi on all three axes).  The reference avoids that point by
+// nudging k by +1e-4 (band_structure_woodpile_Lanczos_n60_gamma0p2.m:50-52).  Flags a divisor below
+// 1e-12 of the largest |lambda| (relative squared value 1e-24).
+int check_theta(int n1, int n2, int n3, int nb, const double* theta, const double* delta) {
+  const int dims[3] = {n1, n2, n3};
+  double scale = 0;
+  for (int d = 0; d < 3; ++d) {
+    if (!(delta[d] > 0) || !std::isfinite(delta[d])) return set_err(FAME_ERR_ARG, "delta must be positive and finite");
+    scale += 4.0 / (delta[d] * delta[d]);
+  }
+  const double ws = 0.26726124191242438468, w[3] = {ws, 2 * ws, 3 * ws};
+  std::vector<double2> lam[3];
+  for (int b = 0; b < nb; ++b) {
+    for (int d = 0; d < 3; ++d) {
+      const double th = theta[3 * b + d];
+      if (!std::isfinite(th)) return set_err(FAME_ERR_ARG, "theta must be finite");
+      lam[d].resize(dims[d]);
+      for (int j = 0; j < dims[d]; ++j) {
+        const double arg = 2.0 * ((double)j * M_PI / (double)dims[d]) + th / (double)dims[d];
+        lam[d][j] = make_double2((std::cos(arg) - 1.0) / delta[d], std::sin(arg) / delta[d]);
+      }
+    }
+    double worst = 1e300;
+    for (int j3 = 0; j3 < n3; ++j3)
+      for (int j2 = 0; j2 < n2; ++j2)
+        for (int j1 = 0; j1 < n1; ++j1) {
+          const double2 a = lam[0][j1], c = lam[1][j2], e = lam[2][j3];
+          const double lq = a.x * a.x + a.y * a.y + c.x * c.x + c.y * c.y + e.x * e.x + e.y * e.y;
+          const double wx = w[0] * a.x + w[1] * c.x + w[2] * e.x, wy = w[0] * a.y + w[1] * c.y + w[2] * e.y;
+          worst = std::min(worst, lq - (wx * wx + wy * wy));
+        }
+    if (!(worst > 1e-24 * scale)) {
+      char msg[320];
+      std::snprintf(msg, sizeof msg,
+                    "k-point %d (theta = %.17g, %.17g, %.17g): the curl SVD is singular at a Fourier point "
+                    "(Lambda_q - |w.lambda|^2 = %.3g); at Gamma nudge k by +1e-4 as "
+                    "band_structure_woodpile_Lanczos_n60_gamma0p2.m:50-52 does",
+                    b, theta[3 * b], theta[3 * b + 1], theta[3 * b + 2], worst);
+      return set_err(FAME_ERR_ARG, msg);
+    }
+  }
+  return FAME_OK;
+}
+
 int upload_k_tables(fame_plan* P) {
   std::vector<double2> lam, tws, om;
   fill_k_tables(P, lam, tws, om);
@@ -849,6 +896,7 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   *plan = nullptr;
   if (n1 < 1 || n2 < 1 || n3 < 1 || nb < 1) return set_err(FAME_ERR_ARG, "grid dims and nb must be >= 1");
   if (!theta || !delta) return set_err(FAME_ERR_ARG, "theta and delta are required");
+  RET_IF(check_theta(n1, n2, n3, nb, theta, delta));
   if (n_mat < 1 || n_mat > kMaxMat || !mat_table)
     return set_err(FAME_ERR_ARG, "need 1 <= n_mat <= " + std::to_string(kMaxMat) + " and a material table");
   if ((long long)n1 * n2 * (long long)sizeof(double2) > 160 * 1024)
@@ -962,6 +1010,7 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
 
 int fame_plan_set_theta(fame_plan* P, const double* theta) {
   if (!P || !theta) return set_err(FAME_ERR_ARG, "NULL argument");
+  RET_IF(check_theta(P->n1, P->n2, P->n3, P->nb, theta, P->delta));
   P->theta.assign(theta, theta + 3 * P->nb);
   return upload_k_tables(P);
 }

@@ -111,6 +111,9 @@ def load_library(path: str = LIB_PATH):
         "fame_fcc_Qrs": ([vp, vp, vp], i),
         "fame_fcc_nfgep_A": ([vp, vp, vp, vp, vp, vp], i),
         "fame_fcc_eigenvector": ([vp, vp, vp, vp, vp, vp], i),
+        "fame_fcc_nfgep_A_ex": ([vp, vp, vp, vp, vp, vp, i], i),
+        "fame_fcc_eigenvector_ex": ([vp, vp, vp, vp, vp, vp, i], i),
+        "fame_fcc_nfgep_data": ([vp, vp, vp, vp], i),
         "fame_fcc_set_material": ([vp, vp, vp, pd], i),
         "fame_fcc_param_apply": ([vp, i, vp, vp], i),
         "fame_fcc_gep_B": ([vp, vp, vp], i),
@@ -137,7 +140,8 @@ def exported_symbols():
         "fame_plan_set_vertex_material", "fame_material_gradient",
         "fame_fcc_create", "fame_fcc_destroy", "fame_fcc_set_stream", "fame_fcc_sync", "fame_fcc_set_generic", "fame_fcc_T", "fame_fcc_TH",
         "fame_fcc_Sigma_r", "fame_fcc_Pr", "fame_fcc_Qr", "fame_fcc_Prs", "fame_fcc_Qrs", "fame_fcc_nfgep_A",
-        "fame_fcc_eigenvector", "fame_fcc_set_material", "fame_fcc_param_apply", "fame_fcc_gep_B",
+        "fame_fcc_eigenvector", "fame_fcc_nfgep_A_ex", "fame_fcc_eigenvector_ex", "fame_fcc_nfgep_data",
+        "fame_fcc_set_material", "fame_fcc_param_apply", "fame_fcc_gep_B",
     ]
 
 
@@ -600,24 +604,33 @@ class FccPlan:
             return t.full((24 * self.n,), complex(c), dtype=t.complex128, device=self.device)
         return self._dev(c, 24 * self.n)
 
-    def mtx_prod_vec_A_NFGEP(self, x, xi_d, zeta_d, Phi):
+    def mtx_prod_vec_A_NFGEP(self, x, xi_d, zeta_d, Phi, coupling=0):
         """mtx_prod_vec_A_NFGEP (mtx_prod_vec_A_NFGEP.m:1) with diagonal Phi; xi_d, zeta_d, Phi are
-        scalars or 24n vectors."""
+        scalars or 24n vectors: the nonzeros of mtx_data.xi_d / zeta_d, on the diagonal (coupling 0)
+        or in the other 12n half (coupling 1, the form ``nfgep_data`` returns)."""
         xin = self._dev(x, 32 * self.n)
         c = [self._coef(v) for v in (xi_d, zeta_d, Phi)]
         y = self._empty(32 * self.n)
-        _check(self.lib.fame_fcc_nfgep_A(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr()),
-                                         *(ctypes.c_void_p(v.data_ptr()) for v in c)))
+        _check(self.lib.fame_fcc_nfgep_A_ex(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                            *(ctypes.c_void_p(v.data_ptr()) for v in c), int(coupling)))
         return y
 
-    def Compute_eigenvector(self, x, xi_d, zeta_d, Phi):
+    def Compute_eigenvector(self, x, xi_d, zeta_d, Phi, coupling=0):
         """Compute_eigenvector (Compute_eigenvector.m:1) with diagonal Phi: 32n -> 48n."""
         xin = self._dev(x, 32 * self.n)
         c = [self._coef(v) for v in (xi_d, zeta_d, Phi)]
         y = self._empty(48 * self.n)
-        _check(self.lib.fame_fcc_eigenvector(self.h, ctypes.c_void_p(xin.data_ptr()), ctypes.c_void_p(y.data_ptr()),
-                                             *(ctypes.c_void_p(v.data_ptr()) for v in c)))
+        _check(self.lib.fame_fcc_eigenvector_ex(self.h, ctypes.c_void_p(xin.data_ptr()),
+                                                ctypes.c_void_p(y.data_ptr()),
+                                                *(ctypes.c_void_p(v.data_ptr()) for v in c), int(coupling)))
         return y
+
+    def nfgep_data(self):
+        """(xi_d, zeta_d, Phi) of the NFGEP for the material set by mtx_Eps_Mu_Zeta_Xi, derived from
+        GEP B (Construct_GEP_mtx_B.m; mu = I, diagonal eps, zeta, xi), for coupling = 1."""
+        out = [self._empty(24 * self.n) for _ in range(3)]
+        _check(self.lib.fame_fcc_nfgep_data(self.h, *(ctypes.c_void_p(v.data_ptr()) for v in out)))
+        return tuple(out)
 
     @staticmethod
     def masks(n, sets):

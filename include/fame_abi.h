@@ -273,6 +273,24 @@ int fame_fcc_nfgep_A(fame_fcc* plan, const void* in, void* out, const void* xi_d
  * (light_locked_Huang/Compute_eigenvector.m:1-43), diagonal Phi. */
 int fame_fcc_eigenvector(fame_fcc* plan, const void* in, void* out, const void* xi_d, const void* zeta_d,
                          const void* Phi);
+/* The same two maps (mtx_prod_vec_A_NFGEP.m:12-50, Compute_eigenvector.m:9-41) with the
+ * sparsity of mtx_data.xi_d and mtx_data.zeta_d (24n x 24n matrices in the reference, applied
+ * as `mtx_data.xi_d * tmp1`) stated by `coupling`: one nonzero per row, given by the 24n arrays
+ * xi_d / zeta_d, on the diagonal (coupling = 0: the calls above) or in the other 12n half
+ * (coupling = 1: (xi_d t)[r] = xi_d[r] t[(r + 12n) mod 24n]), which is the form the physical
+ * GEP B takes (fame_fcc_nfgep_data). */
+int fame_fcc_nfgep_A_ex(fame_fcc* plan, const void* in, void* out, const void* xi_d, const void* zeta_d,
+                        const void* Phi, int coupling);
+int fame_fcc_eigenvector_ex(fame_fcc* plan, const void* in, void* out, const void* xi_d, const void* zeta_d,
+                            const void* Phi, int coupling);
+/* mtx_data (xi_d, zeta_d, Phi; 24n complex device arrays each, coupling = 1) for the material of
+ * fame_fcc_set_material, derived from GEP B of Construct_GEP_mtx_B.m:1-69.  The reference never
+ * builds mtx_data; this is the one for which Compute_eigenvector equals its commented check
+ * `GEP_mtx_B \ [tmp1; tmp2]` (Compute_eigenvector.m:27,41) after the similarity diag(I, -iI) B
+ * diag(I, iI), which leaves GEP A unchanged, so the NFGEP spectrum is the nonzero spectrum of
+ * (GEP A, GEP B).  Needs mu = I and diagonal eps, zeta, xi (inside and air), else
+ * FAME_ERR_UNSUPPORTED (Phi would not be diagonal: the LU branch of mtx_prod_vec_A_NFGEP.m:29-30). */
+int fame_fcc_nfgep_data(fame_fcc* plan, void* xi_d, void* zeta_d, void* Phi);
 /* Material of mtx_Eps_Mu_Zeta_Xi (light_locked_Huang/mtx_Eps_Mu_Zeta_Xi.m:1-38): host arrays.
  * edge_mask[n]: bit 0/1/2 = point in edge_idx_in.X/Y/Z, bit 3 = in Cubic_idx_in;
  * face_mask[n]: bit 0/1/2 = face_idx_in.X/Y/Z, bit 3 = vertex_idx_in.

@@ -16,6 +16,8 @@ root, MATLAB column-major reshapes written out with ``order="F"``):
 * ``gep_B``                       light_locked_Huang/Construct_GEP_mtx_B.m:1-69
 * ``nfgep_A_matvec``              light_locked_Huang/mtx_prod_vec_A_NFGEP.m:1-52 (diagonal Phi)
 * ``compute_eigenvector``         light_locked_Huang/Compute_eigenvector.m:1-43 (diagonal Phi)
+* ``nfgep_data``, ``fncA``        mtx_data and the NFGEP right-hand operator, derived from
+                                  Construct_GEP_mtx_A/B.m (the reference never builds them)
 
 Pinning (tests/test_fcc_oracle_pins.py): T is unitary; T equals the closed-form
 triple sum of the twisted inverse DFTs; the explicit sparse difference matrices
@@ -23,9 +25,11 @@ are diagonalised by T (Gamma_d T = delta_d T diag(Lambda_d), two independent
 reference files agreeing); every curl block factors as
 C_v (I_3 (x) T) Q_v = (I_3 (x) T) P_v diag(Sigma_v); the parameter matrices are
 four permuted copies of pointwise 3x3 tensors (in/air by index set).
-``nfgep_A_matvec``: the reference never builds its ``mtx_data`` (xi_d, zeta_d,
-Phi), so it is pinned only by algebra (block form and Hermiticity for real Phi,
-zeta_d = conj(xi_d)).
+``nfgep_A_matvec`` / ``compute_eigenvector``: the reference never builds its ``mtx_data``;
+``nfgep_data`` derives it from GEP B, and then (tests/test_fcc_oracle_pins.py)
+compute_eigenvector equals a sparse solve with that B (the reference's own commented check),
+the NFGEP spectrum equals the nonzero spectrum of the explicit (GEP A, GEP B), and the mapped
+NFGEP eigenvectors are GEP eigenvectors.
 """
 from __future__ import annotations
 
@@ -306,26 +310,71 @@ def _apply_chunks(f, v, n):
     return np.concatenate([f(v[i * n:(i + 1) * n]) for i in range(v.size // n)])
 
 
-def nfgep_A_matvec(x, tab, svd, xi_d, zeta_d, Phi):
-    """mtx_prod_vec_A_NFGEP.m:12-50 with mtx_data.Phi_is_diag = 'yes'; xi_d, zeta_d are the
-    diagonals (24n, or scalars) of mtx_data.xi_d / zeta_d."""
+def _couple(c, v, coupling):
+    """mtx_data.xi_d * v (or zeta_d * v) for a 24n x 24n matrix with one nonzero per row: the
+    24n array ``c`` on the diagonal (coupling 0) or in the other 12n half (coupling 1)."""
+    if coupling == 0:
+        return c * v
+    h = v.size // 2
+    return c * np.concatenate([v[h:], v[:h]])
+
+
+def nfgep_A_matvec(x, tab, svd, xi_d, zeta_d, Phi, coupling=0):
+    """mtx_prod_vec_A_NFGEP.m:12-50 with mtx_data.Phi_is_diag = 'yes'.  xi_d, zeta_d: the nonzeros
+    (24n, or scalars) of mtx_data.xi_d / zeta_d, diagonal (coupling 0) or coupling the 12n halves
+    (coupling 1, the form ``nfgep_data`` derives from GEP B)."""
     n = svd["n"]
     tmp1 = _apply_chunks(lambda v: T(v, tab), svd["Lambda_Pr"] @ x[:16 * n], n)
     tmp2 = _apply_chunks(lambda v: T(v, tab), svd["Lambda_Qr"] @ x[16 * n:], n)
-    vec1 = (xi_d * tmp1 + tmp2) / Phi
-    tmp1 = tmp1 + zeta_d * vec1
+    vec1 = (_couple(xi_d, tmp1, coupling) + tmp2) / Phi
+    tmp1 = tmp1 + _couple(zeta_d, vec1, coupling)
     out1 = svd["Lambda_Pr"].conj().T @ _apply_chunks(lambda v: TH(v, tab), tmp1, n)
     out2 = svd["Lambda_Qr"].conj().T @ _apply_chunks(lambda v: TH(v, tab), vec1, n)
     return np.concatenate([out1, out2])
 
 
-def compute_eigenvector(x, tab, svd, xi_d, zeta_d, Phi):
+def compute_eigenvector(x, tab, svd, xi_d, zeta_d, Phi, coupling=0):
     """Compute_eigenvector.m:9-37 (diagonal Phi): ev = -i [-vec1; tmp1 + zeta_d vec1] (48n)."""
     n = svd["n"]
     tmp1 = _apply_chunks(lambda v: T(v, tab), svd["Lambda_Pr"] @ x[:16 * n], n)
     tmp2 = _apply_chunks(lambda v: T(v, tab), svd["Lambda_Qr"] @ x[16 * n:], n)
-    vec1 = (xi_d * tmp1 + tmp2) / Phi
-    return -1j * np.concatenate([-vec1, tmp1 + zeta_d * vec1])
+    vec1 = (_couple(xi_d, tmp1, coupling) + tmp2) / Phi
+    return -1j * np.concatenate([-vec1, tmp1 + _couple(zeta_d, vec1, coupling)])
+
+
+def nfgep_data(m):
+    """mtx_data (xi_d, zeta_d, Phi; coupling 1) from the eight parameter matrices of
+    mtx_Eps_Mu_Zeta_Xi (mu = I, diagonal eps, zeta, xi).  With the 24n blocks of GEP B
+    (Construct_GEP_mtx_B.m) B11 = [[0, Zf], [Ze, 0]], B12 = I, B21 = -diag(Ee, Ef),
+    B22 = -[[0, Xe], [Xf, 0]], the similarity diag(I, -iI) B diag(I, iI) (GEP A unchanged) is
+    i [[zeta_d, I], [-(Phi + xi_d zeta_d), -xi_d]], the B for which Compute_eigenvector returns
+    B \\ [tmp1; tmp2] (its commented check, Compute_eigenvector.m:27,41):
+    xi_d = i B22, zeta_d = -i B11, Phi = B21 - xi_d zeta_d."""
+    diag = {}
+    for k, M in m.items():
+        M = sp.csr_matrix(M)
+        if k.startswith("Mu"):
+            if abs(M - sp.identity(M.shape[0])).max() != 0:
+                raise ValueError("nfgep_data needs mu = I")
+        elif abs(M - sp.diags(M.diagonal())).max() != 0:
+            raise ValueError("nfgep_data needs diagonal eps, zeta, xi (Phi is not diagonal otherwise)")
+        diag[k] = M.diagonal()
+    Ee, Ef, Ze, Zf, Xe, Xf = (diag[k] for k in ("Eps_e", "Eps_f", "Zeta_e", "Zeta_f", "Xi_e", "Xi_f"))
+    xi_d = np.concatenate([-1j * Xe, -1j * Xf])
+    zeta_d = np.concatenate([-1j * Zf, -1j * Ze])
+    Phi = np.concatenate([-Ee + Xe * Ze, -Ef + Xf * Zf])
+    return xi_d, zeta_d, Phi
+
+
+def fncA(x, svd):
+    """The NFGEP's right-hand operator [i Sigma^-1 x_b; -i Sigma^-1 x_a] (32n), Sigma =
+    [Sigma_r; Sigma_r] on each half: with GEP A = blkdiag(P_r Sigma Q_r^H, Q_r Sigma P_r^H)
+    (Construct_GEP_mtx_A.m:314-320 through the SVD of SVDSingleCurl_FCC.m), the nonzero eigenvalues
+    of (GEP A, GEP B) are those of mtx_prod_vec_A_NFGEP z = lambda fncA z (the FCC twin of
+    fun_Lanczos/fncA.m:3-4)."""
+    n = svd["n"]
+    s = np.concatenate([svd["Sigma_r"], svd["Sigma_r"]])
+    return np.concatenate([1j * x[16 * n:] / s, -1j * x[:16 * n] / s])
 
 
 # lattice of the skewed FCC grid (Construct_idx_all_FCC.m:150-155): a1 = (1,0,0),

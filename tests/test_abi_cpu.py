@@ -40,6 +40,34 @@ def test_plan_create_rejects_bad_args_without_gpu(pkg):
     assert lib.fame_plan_create(ctypes.byref(h), 4, 4, 4, 1, th, de, 0, tab, None) == -1
 
 
+def test_plan_create_rejects_gamma_point_without_gpu(pkg):
+    """theta = 0 mod 2 pi on all three axes makes Lambda_q = 0 at the zero frequency, where
+    Pi_0 / Pi_1 / Pi_2 (SVD_Matrix_curl.m:45-57) divide by zero; the plan is refused on the host,
+    before any device call, with a message naming the reference's k nudge
+    (band_structure_woodpile_Lanczos_n60_gamma0p2.m:50-52).  The nudged k and a generic k pass
+    the check (and then fail only for want of a GPU here)."""
+    lib = pkg.load_library()
+    h = ctypes.c_void_p()
+    de = (ctypes.c_double * 3)(1 / 8, 1 / 8, 1 / 8)
+    tab = (ctypes.c_double * 54)()
+    for th in [(0.0, 0.0, 0.0), (2 * np.pi, -2 * np.pi, 4 * np.pi)]:
+        theta = (ctypes.c_double * 6)(0.4, 0.2, 0.1, *th)   # item 1 of 2 is at Gamma
+        assert lib.fame_plan_create(ctypes.byref(h), 8, 8, 8, 2, theta, de, 1, tab, None) == -1
+        msg = lib.fame_last_error()
+        assert b"k-point 1" in msg and b"50-52" in msg and b"nudge" in msg
+    # Pi_1's divisor vanishes where lambda is parallel to w = (1, 2, 3)/sqrt(14) without lambda = 0:
+    # equal theta_d with delta = (1, 1/2, 1/3) gives lambda_d(0) = (e^{i theta/8} - 1) (1, 2, 3)
+    theta = (ctypes.c_double * 3)(0.3, 0.3, 0.3)
+    skew = (ctypes.c_double * 3)(1.0, 0.5, 1 / 3)
+    assert lib.fame_plan_create(ctypes.byref(h), 8, 8, 8, 1, theta, skew, 1, tab, None) == -1
+    assert b"singular" in lib.fame_last_error()
+    S = pkg.structures
+    for k in (np.zeros(3), np.array([0.3, 0.2, 0.1])):
+        theta = (ctypes.c_double * 3)(*S.theta_of_k(k, S.SC_LATTICE))
+        rc = lib.fame_plan_create(ctypes.byref(h), 8, 8, 8, 1, theta, de, 1, tab, None)
+        assert rc != -1 or b"singular" not in lib.fame_last_error()
+
+
 def test_material_slots_match_oracle(pkg, fo):
     S = pkg.structures
     for A in (S.SC_LATTICE, S.FCC_LATTICE):

@@ -8,12 +8,25 @@ fixes, independent of the restated reshapes:
   by T with the Lambda's of SVDSingleCurl_FCC.m (two reference files agreeing);
 * each curl block factors as C_v (I_3 (x) T) Q_v = (I_3 (x) T) P_v diag(Sigma_v), with
   Q_v, P_v orthonormal;
-* the parameter matrices are four permuted copies of pointwise 3x3 tensors.
+* the 36 (tensor entry, index set) pairs of the parameter matrices follow from where the
+  four staggered Yee grids sample each component, and equal the table read from the
+  reference file;
+* with the mtx_data that GEP B fixes (nfgep_data), compute_eigenvector is a solve with that B
+  (the reference's own commented check), the NFGEP spectrum is the nonzero spectrum of the
+  explicit (GEP A, GEP B), and mapped NFGEP eigenvectors solve the GEP.
 """
+import json
+import os
+
 import numpy as np
 import pytest
+import scipy.linalg as sl
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
 
 from oracle import fcc_oracle as fc
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 SIZES = [(2, 3, 3), (4, 6, 6), (6, 3, 4)]
 KS = [np.array([0.31, -0.17, 0.23]), np.array([1.3, 0.4, -0.9])]
@@ -94,37 +107,75 @@ def test_curl_blocks_factor(dims):
     assert not LQr[:12 * n, 8 * n:].any() and not LPr[12 * n:, :8 * n].any()
 
 
-def test_parameter_matrix_is_four_permuted_pointwise_copies():
+def _group_shifts():
+    """Half-cell shift of each of the four Yee grids, read off the curl variants
+    (SVDSingleCurl_FCC.m:33-36): a variant whose Lambda_d is -conj(Lambda_d) takes backward
+    differences along d, i.e. its grid sits half a cell further along d."""
+    L = (1 + 2j, 3 + 4j, 5 + 6j)
+    tab = {k: np.array([v]) for k, v in zip(("Lambda_x", "Lambda_y", "Lambda_z"), L)}
+    shifts = []
+    for var in fc.variants(tab):
+        sh = []
+        for d in range(3):
+            if var[d][0] == L[d]:
+                sh.append(0.0)
+            else:
+                assert var[d][0] == -np.conj(L[d])
+                sh.append(0.5)
+        shifts.append(np.array(sh))
+    return shifts
+
+
+def _derived_param_table():
+    """(row block, col block) -> (tensor row, tensor col, index set) from Yee-grid geometry alone:
+    block 3g + a is component a of grid g; E_a sits on a-edges, H_a on a-faces, shifted by the
+    grid's half-cell offset.  The material couples components sampled at one point, and that
+    point's kind names the index set (edges X/Y/Z or cube centres for the E matrices, faces
+    X/Y/Z or vertices for the H matrices; mtx_Eps_Mu_Zeta_Xi.m:23-35)."""
+    sh = _group_shifts()
+    e = 0.5 * np.eye(3)
+    locE = lambda b: tuple(np.mod(e[b % 3] + sh[b // 3], 1.0))  # noqa: E731
+    locH = lambda b: tuple(np.mod(0.5 - e[b % 3] + sh[b // 3], 1.0))  # noqa: E731
+    edge = [(0.5, 0, 0), (0, 0.5, 0), (0, 0, 0.5), (0.5, 0.5, 0.5)]
+    face = [(0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0), (0, 0, 0)]
+    table = {}
+    for rb in range(12):
+        for cb in range(12):
+            if locE(rb) == locE(cb):
+                s = edge.index(locE(rb))
+                assert face.index(locH(rb)) == s and locH(rb) == locH(cb)
+                table[(rb, cb)] = (rb % 3, cb % 3, s)
+    return table
+
+
+def test_parameter_table_from_yee_geometry():
+    """Every one of the 36 (tensor entry, index set) pairs of Construct_parameter_mtx.m:13-312
+    follows from where the four staggered grids sample each field component."""
+    table = _derived_param_table()
+    assert len(table) == 36
+    assert {(rb, cb): (a, b, s) for rb, cb, a, b, s in fc.PARAM_BLOCKS} == table
+
+
+def test_parameter_table_matches_reference_text():
+    """PARAM_BLOCKS equals, entry by entry and in order, the table read mechanically from the
+    reference file by tests/golden/extract_param_blocks.py."""
+    ref = json.load(open(os.path.join(GOLDEN, "param_blocks.json")))
+    assert [tuple(e) for e in ref["entries"]] == list(fc.PARAM_BLOCKS)
+
+
+def test_parameter_matrix_entries():
+    """parameter_mtx places T_in[a, b] (inside the entry's index set) or T_air[a, b] at every
+    (row block, col block, point) the geometry names, and nothing else."""
     rng = np.random.default_rng(9)
     n = 30
     sets = [rng.random(n) < 0.4 for _ in range(4)]
     Tin = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
     Tair = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
-    M = fc.parameter_mtx(n, sets, Tin, Tair).toarray()
-    assert np.count_nonzero(M) == 36 * n
-    # group the 12 n-blocks into copies: blocks coupled by a nonzero entry
-    blk = (np.abs(M.reshape(12, n, 12, n)).sum(axis=(1, 3)) > 0)
-    copies, seen = [], set()
-    for b in range(12):
-        if b in seen:
-            continue
-        c = sorted(np.nonzero(blk[b])[0])
-        assert len(c) == 3 and all(sorted(np.nonzero(blk[j])[0]) == c for j in c)
-        copies.append(c)
-        seen |= set(c)
-    assert sorted(sum(copies, [])) == list(range(12))
-    # per point: each copy's 3x3 block, in tensor order, is T_in or T_air (one set per copy)
-    n_in = 0
-    for c in copies:
-        for p in range(n):
-            B = M[np.ix_([b * n + p for b in c], [b * n + p for b in c])]
-            # find the component order that makes B equal one of the tensors
-            import itertools
-            hits = [(perm, which) for perm in itertools.permutations(range(3)) for which, Tt in (("in", Tin), ("air", Tair))
-                    if np.array_equal(B[np.ix_(perm, perm)], Tt)]
-            assert hits, (c, p)
-            n_in += hits[0][1] == "in"
-    assert 0 < n_in < 4 * n
+    M = fc.parameter_mtx(n, sets, Tin, Tair).toarray().reshape(12, n, 12, n)
+    want = np.zeros_like(M)
+    for (rb, cb), (a, b, s) in _derived_param_table().items():
+        want[rb, np.arange(n), cb, np.arange(n)] = np.where(sets[s], Tin[a, b], Tair[a, b])
+    assert np.array_equal(M, want)
 
 
 def test_gep_B_blocks():
@@ -159,3 +210,117 @@ def test_nfgep_hermitian_for_real_phi():
     y1 = fc.nfgep_A_matvec(x, tab, svd, 0.3 - 0.2j, 0.3 + 0.2j, 2.0)
     y2 = fc.nfgep_A_matvec(x, tab, svd, np.full(24 * n, 0.3 - 0.2j), np.full(24 * n, 0.3 + 0.2j), np.full(24 * n, 2.0))
     assert np.allclose(y1, y2, rtol=1e-14, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# NFGEP against the explicit GEP (Construct_GEP_mtx_A.m, Construct_GEP_mtx_B.m)
+# ---------------------------------------------------------------------------
+
+def fcc_case(dims, k, seed=11, gamma=0.7, eps_in=(13.0, 13.0, 13.0)):
+    """Random inside/air index sets; inside eps = diag(eps_in), mu = I, zeta = -i gamma I,
+    xi = i gamma I (BASELINE's bi-isotropic inclusions), vacuum outside."""
+    tab, (a1, a2, a3), delta = setup(dims, k)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(seed)
+    es = [rng.random(n) < 0.5 for _ in range(4)]
+    fs = [rng.random(n) < 0.5 for _ in range(4)]
+    I3 = np.eye(3)
+    tin = [np.diag(eps_in).astype(complex), I3, -1j * gamma * I3, 1j * gamma * I3]
+    tair = [I3, I3, 0 * I3, 0 * I3]
+    m = fc.eps_mu_zeta_xi(n, es, fs, tin, tair)
+    blocks, _ = fc.curl_blocks(*dims, k, a1, a2, a3, delta)
+    A, _ = fc.gep_A(blocks)
+    return tab, fc.svd_curl(tab), m, A, fc.gep_B(m), n
+
+
+def similarity(n):
+    """R, P with R A P = A for GEP A (blocks scaled by 1 and -i*i) and R B P of the NFGEP form."""
+    R = sp.block_diag([sp.identity(24 * n), -1j * sp.identity(24 * n)]).tocsr()
+    P = sp.block_diag([sp.identity(24 * n), 1j * sp.identity(24 * n)]).tocsr()
+    return R, P
+
+
+@pytest.mark.parametrize("dims", [(2, 3, 3), (4, 3, 3)])
+def test_eigenvector_map_is_gep_B_solve(dims):
+    """Compute_eigenvector.m:27,41: ev equals GEP_mtx_B \\ [P_r u; Q_r v] for the B whose
+    similarity transform the mtx_data comes from."""
+    tab, svd, m, A, B, n = fcc_case(dims, KS[0])
+    xi, zeta, Phi = fc.nfgep_data(m)
+    R, P = similarity(n)
+    assert abs(R @ A @ P - A).max() == 0
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(32 * n) + 1j * rng.standard_normal(32 * n)
+    ev = fc.compute_eigenvector(x, tab, svd, xi, zeta, Phi, coupling=1)
+    Tm = dense(lambda v: fc.T(v, tab), n)
+    I24T = sp.kron(sp.identity(24), sp.csr_matrix(Tm))
+    t = np.concatenate([I24T @ (svd["Lambda_Pr"] @ x[:16 * n]), I24T @ (svd["Lambda_Qr"] @ x[16 * n:])])
+    ref = spl.spsolve((R @ B @ P).tocsc(), t)
+    assert np.linalg.norm(ev - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def nfgep_dense(tab, svd, data, n, coupling=1):
+    NF = dense(lambda v: fc.nfgep_A_matvec(v, tab, svd, *data, coupling=coupling), 32 * n)
+    AF = dense(lambda v: fc.fncA(v, svd), 32 * n)
+    return NF, AF
+
+
+def match(w_small, w_big):
+    """largest relative distance from each of w_small to the nearest of w_big"""
+    return max(np.min(np.abs(w_big - w)) / abs(w) for w in w_small)
+
+
+@pytest.mark.parametrize("ki,eps_in", [(0, (13.0, 13.0, 13.0)), (1, (12.0, 13.0, 14.0))])
+def test_nfgep_spectrum_is_gep_spectrum(ki, eps_in):
+    """The nonzero eigenvalues of the explicit (GEP A, GEP B), 48n of them less the 16n of A's
+    null space, are exactly the NFGEP eigenvalues (mtx_prod_vec_A_NFGEP z = lambda fncA z)."""
+    dims = (2, 3, 3)
+    tab, svd, m, A, B, n = fcc_case(dims, KS[ki], eps_in=eps_in)
+    NF, AF = nfgep_dense(tab, svd, fc.nfgep_data(m), n)
+    w_nf = sl.eigvals(NF, AF)
+    w_gep = sl.eigvals(A.toarray(), B.toarray())
+    w_gep = w_gep[np.abs(w_gep) > 1e-9 * np.abs(w_gep).max()]
+    assert w_nf.size == w_gep.size == 32 * n
+    assert match(w_nf, w_gep) < 1e-10 and match(w_gep, w_nf) < 1e-10
+
+
+def test_nfgep_eigenvectors_solve_the_gep():
+    """For NFGEP eigenpairs (lambda, z), x = diag(I, iI) Compute_eigenvector(z) satisfies
+    GEP_A x = lambda GEP_B x."""
+    dims = (2, 3, 3)
+    tab, svd, m, A, B, n = fcc_case(dims, KS[1], gamma=1.1)
+    data = fc.nfgep_data(m)
+    NF, AF = nfgep_dense(tab, svd, data, n)
+    w, Z = sl.eig(NF, AF)
+    _, P = similarity(n)
+    order = np.argsort(np.abs(w))
+    for i in list(order[:4]) + list(order[-2:]):
+        x = P @ fc.compute_eigenvector(Z[:, i], tab, svd, *data, coupling=1)
+        res = np.linalg.norm(A @ x - w[i] * (B @ x)) / (np.linalg.norm(x) * abs(w[i]))
+        assert res < 1e-11, (i, w[i], res)
+
+
+def test_nfgep_pin_detects_planted_mistakes():
+    """A swap of Lambda_Pr / Lambda_Qr, of xi_d / zeta_d, or the diagonal coupling instead of the
+    half coupling, planted in the oracle's inputs, moves the spectrum off the GEP's."""
+    dims = (2, 3, 3)
+    tab, svd, m, A, B, n = fcc_case(dims, KS[0])
+    xi, zeta, Phi = fc.nfgep_data(m)
+    w_gep = sl.eigvals(A.toarray(), B.toarray())
+    w_gep = w_gep[np.abs(w_gep) > 1e-9 * np.abs(w_gep).max()]
+    swapped = dict(svd, Lambda_Pr=svd["Lambda_Qr"], Lambda_Qr=svd["Lambda_Pr"])
+    for s_, data, coupling in [(swapped, (xi, zeta, Phi), 1), (svd, (zeta, xi, Phi), 1), (svd, (xi, zeta, Phi), 0)]:
+        NF, AF = nfgep_dense(tab, s_, data, n, coupling)
+        assert match(sl.eigvals(NF, AF), w_gep) > 1e-3
+
+
+def test_nfgep_data_rejects_what_it_cannot_represent():
+    dims = (2, 3, 3)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(1)
+    es = [rng.random(n) < 0.5 for _ in range(4)]
+    I3 = np.eye(3)
+    aniso = np.array([[5, 2, 1], [2, 5, 2], [1, 2, 4]], complex)
+    with pytest.raises(ValueError):
+        fc.nfgep_data(fc.eps_mu_zeta_xi(n, es, es, [aniso, I3, 0 * I3, 0 * I3], [I3, I3, 0 * I3, 0 * I3]))
+    with pytest.raises(ValueError):
+        fc.nfgep_data(fc.eps_mu_zeta_xi(n, es, es, [I3, 2 * I3, 0 * I3, 0 * I3], [I3, I3, 0 * I3, 0 * I3]))

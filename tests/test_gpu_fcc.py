@@ -194,3 +194,113 @@ def test_specialised_T_equals_generic(pkg, torch_mod, n):
     assert float((yf - yg).norm() / yg.norm()) < 1e-13
     assert float((zf - zg).norm() / zg.norm()) < 1e-13
     plan.close()
+
+
+# ---------------------------------------------------------------------------
+# NFGEP with the mtx_data that GEP B fixes (half-coupled xi_d / zeta_d): the oracle's
+# maps are pinned against the explicit (GEP A, GEP B) in tests/test_fcc_oracle_pins.py
+# ---------------------------------------------------------------------------
+
+def physical_material(pkg, plan, n, gamma=0.7, eps_in=(12.0, 13.0, 14.0), seed=7):
+    r = np.random.default_rng(seed)
+    edge = [r.random(n) < 0.4 for _ in range(4)]
+    face = [r.random(n) < 0.4 for _ in range(4)]
+    I3 = np.eye(3)
+    tin = [np.diag(eps_in).astype(complex), I3, -1j * gamma * I3, 1j * gamma * I3]
+    tair = [I3, I3, 0 * I3, 0 * I3]
+    plan.mtx_Eps_Mu_Zeta_Xi(edge, face, tin, tair)
+    return fc.eps_mu_zeta_xi(n, edge, face, tin, tair)
+
+
+@pytest.mark.parametrize("dims", [(2, 3, 3), (4, 6, 6), (8, 8, 8), (12, 12, 12)])
+def test_nfgep_physical_data_matches_oracle(pkg, torch_mod, dims):
+    """fame_fcc_nfgep_data = the oracle's nfgep_data; the half-coupled NFGEP matvec and
+    eigenvector map (generic kernels, and the fused ones at 8^3 and 12^3) = the oracle's."""
+    say(f"FCC NFGEP with GEP-B mtx_data dims={dims}")
+    plan, tab = make(pkg, dims, K)
+    svd = fc.svd_curl(tab)
+    n = plan.n
+    m = physical_material(pkg, plan, n)
+    xi, zeta, phi = (v.cpu().numpy() for v in plan.nfgep_data())
+    rx, rz, rp = fc.nfgep_data(m)
+    assert rel(xi, rx) < 1e-15 and rel(zeta, rz) < 1e-15 and rel(phi, rp) < 1e-15
+    t = torch_mod
+    x = crand(32 * n)
+    args = [t.as_tensor(v) for v in (xi, zeta, phi)]
+    y = plan.mtx_prod_vec_A_NFGEP(t.as_tensor(x), *args, coupling=1)
+    assert rel(y.cpu().numpy(), fc.nfgep_A_matvec(x, tab, svd, rx, rz, rp, coupling=1)) < RTOL
+    ev = plan.Compute_eigenvector(t.as_tensor(x), *args, coupling=1)
+    assert rel(ev.cpu().numpy(), fc.compute_eigenvector(x, tab, svd, rx, rz, rp, coupling=1)) < RTOL
+    plan.close()
+
+
+def test_nfgep_eigenvector_solves_gep_on_device(pkg, torch_mod):
+    """NFGEP eigenpairs from the device matvec (dense, 2x3x3): their eigenvalues are the nonzero
+    eigenvalues of the explicit (GEP A, GEP B), and the device Compute_eigenvector, mapped by
+    diag(I, iI), solves GEP_A x = lambda GEP_B x."""
+    import scipy.linalg as sl
+    import scipy.sparse as sp
+
+    say("FCC NFGEP eigenpairs on the device vs the explicit GEP, 2x3x3")
+    dims = (2, 3, 3)
+    plan, tab = make(pkg, dims, K)
+    svd = fc.svd_curl(tab)
+    n = plan.n
+    m = physical_material(pkg, plan, n, gamma=1.1)
+    data = plan.nfgep_data()
+    t = torch_mod
+    E = t.eye(32 * n, dtype=t.complex128, device="cuda")
+    NF = np.stack([plan.mtx_prod_vec_A_NFGEP(E[j], *data, coupling=1).cpu().numpy() for j in range(32 * n)], 1)
+    AF = np.stack([fc.fncA(e, svd) for e in np.eye(32 * n, dtype=complex)], 1)
+    w, Z = sl.eig(NF, AF)
+    a1, a2, a3 = fc.fcc_lattice()
+    delta = np.array([1 / 2, np.sqrt(3) / 2 / 3, np.sqrt(6) / 3 / 3])
+    blocks, _ = fc.curl_blocks(*dims, K, a1, a2, a3, delta)
+    A, _ = fc.gep_A(blocks)
+    B = fc.gep_B(m)
+    wg = sl.eigvals(A.toarray(), B.toarray())
+    wg = wg[np.abs(wg) > 1e-9 * np.abs(wg).max()]
+    assert max(np.min(np.abs(wg - x)) / abs(x) for x in w) < 1e-10
+    P = sp.block_diag([sp.identity(24 * n), 1j * sp.identity(24 * n)]).tocsr()
+    for i in np.argsort(np.abs(w))[:6]:
+        ev = plan.Compute_eigenvector(t.as_tensor(Z[:, i]), *data, coupling=1).cpu().numpy()
+        x = P @ ev
+        res = np.linalg.norm(A @ x - w[i] * (B @ x)) / (np.linalg.norm(x) * abs(w[i]))
+        assert res < 1e-11, (w[i], res)
+    plan.close()
+
+
+@pytest.mark.parametrize("n", [8, 24, 48, 60, 64])
+def test_nfgep_coupled_specialised_equals_generic(pkg, torch_mod, n):
+    """The half-coupled NFGEP on the fused path (pairing vector u with 24 + (u + 12) mod 24) equals
+    the generic path to rounding."""
+    say(f"FCC coupled NFGEP specialised == generic n={n}")
+    plan, _ = make(pkg, (n, n, n), K)
+    N = plan.n
+    t = torch_mod
+    dev = dict(device="cuda")
+    x = t.as_tensor(crand(32 * N), **dev)
+    xi, zeta = t.as_tensor(crand(24 * N), **dev), t.as_tensor(crand(24 * N), **dev)
+    phi = t.as_tensor(crand(24 * N) + 3, **dev)
+    yf = plan.mtx_prod_vec_A_NFGEP(x, xi, zeta, phi, coupling=1)
+    ef = plan.Compute_eigenvector(x, xi, zeta, phi, coupling=1)
+    plan.set_generic(True)
+    yg = plan.mtx_prod_vec_A_NFGEP(x, xi, zeta, phi, coupling=1)
+    eg = plan.Compute_eigenvector(x, xi, zeta, phi, coupling=1)
+    assert float((yf - yg).norm() / yg.norm()) < 1e-13
+    assert float((ef - eg).norm() / eg.norm()) < 1e-13
+    plan.close()
+
+
+def test_svd_maps_refuse_gamma_point(pkg, torch_mod):
+    """At k = 0 the FCC curl SVD divides by zero (mode (0,0,0)): T and T^H still run, the SVD
+    maps raise instead of returning Inf/NaN."""
+    say("FCC k=0: SVD maps refused")
+    plan, _ = make(pkg, (4, 6, 6), np.zeros(3))
+    n = plan.n
+    t = torch_mod
+    for call in (lambda: plan.Sigma_r(), lambda: plan.Pr(t.as_tensor(crand(16 * n))),
+                 lambda: plan.mtx_prod_vec_A_NFGEP(t.as_tensor(crand(32 * n)), 0.1, 0.1, 2.0)):
+        with pytest.raises(pkg.FameError, match="Gamma"):
+            call()
+    plan.close()

@@ -450,3 +450,21 @@ def test_batched_lanczos_equals_plain(pkg, torch_mod):
         assert np.allclose(ew1[0], ewb[b], rtol=1e-12, atol=0)
         p1.close()
     pb.close()
+
+
+def test_set_theta_refuses_gamma_point(pkg, torch_mod):
+    """fame_plan_set_theta validates on the host like fame_plan_create: theta = 0 on all axes
+    (Lambda_q = 0 at the zero frequency) is refused with FAME_ERR_ARG, and the plan keeps working
+    with its previous k-points."""
+    say("set_theta at Gamma refused")
+    S = pkg.structures
+    dims = (8, 8, 8)
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)[None]
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, theta, table, ids)
+    x = torch_mod.as_tensor(crand(1, 4 * plan.N))
+    y0 = plan.fncB(x).cpu().numpy()
+    with pytest.raises(pkg.FameError, match="nudge"):
+        plan.set_theta(np.zeros((1, 3)))
+    assert np.array_equal(plan.fncB(x).cpu().numpy(), y0)
+    plan.close()
