@@ -6,4 +6,4 @@ with ``importlib`` under an alias such as ``phc_amd`` — see ``load_package`` i
 """
 from . import adjoint, driver, jdsira, outputs, structures  # noqa: F401
 from .fame import (FameError, FccPlan, NotConverged, Plan, exported_symbols, load_library,  # noqa: F401
-                   LIB_PATH)
+                   stream_probe, LIB_PATH)

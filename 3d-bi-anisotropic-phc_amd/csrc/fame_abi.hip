@@ -67,6 +67,7 @@ bool factor_radices(int n, std::vector<int>& r) {
 struct fame_plan {
   int n1 = 0, n2 = 0, n3 = 0, nb = 0;
   long long N = 0;
+  int device = 0;                    // the HIP device current when the plan was created
   hipStream_t stream = nullptr;      // stream in use
   hipStream_t own_stream = nullptr;  // stream created by the plan
   Geo geo{};
@@ -930,6 +931,7 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
     seen[dims[d]] = d;
   }
   auto fail = [&](int rc) { fame_plan_destroy(P); return rc; };
+  if (hipGetDevice(&P->device) != hipSuccess) return fail(set_err(FAME_ERR_HIP, "hipGetDevice failed"));
 #define PLAN_HIP(call)                                                                              \
   do {                                                                                              \
     hipError_t e_ = (call);                                                                         \
@@ -1326,3 +1328,4 @@ int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit,
 #include "fame_minres.inc"
 #include "fame_fcc.inc"
 #include "fame_vertex.inc"
+#include "fame_probe.inc"

@@ -97,6 +97,7 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
         "fame_plan_set_vertex_material": ([vp, pd, pd, d, pd], i),
         "fame_material_gradient": ([vp, i, vp, vp], i),
+        "fame_stream_probe": ([i, vp, vp, ctypes.c_longlong, i, vp, pd], i),
         "fame_fcc_create": ([ctypes.POINTER(vp), i, i, i, pd, pd, pd, pd, pd], i),
         "fame_fcc_destroy": ([vp], i),
         "fame_fcc_set_stream": ([vp, vp], i),
@@ -137,7 +138,7 @@ def exported_symbols():
         "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
-        "fame_plan_set_vertex_material", "fame_material_gradient",
+        "fame_plan_set_vertex_material", "fame_material_gradient", "fame_stream_probe",
         "fame_fcc_create", "fame_fcc_destroy", "fame_fcc_set_stream", "fame_fcc_sync", "fame_fcc_set_generic", "fame_fcc_T", "fame_fcc_TH",
         "fame_fcc_Sigma_r", "fame_fcc_Pr", "fame_fcc_Qr", "fame_fcc_Prs", "fame_fcc_Qrs", "fame_fcc_nfgep_A",
         "fame_fcc_eigenvector", "fame_fcc_nfgep_A_ex", "fame_fcc_eigenvector_ex", "fame_fcc_nfgep_data",
@@ -160,6 +161,20 @@ def _dptr(arr):
     if arr.dtype != np.float64 or not arr.flags.c_contiguous:
         raise TypeError("_dptr needs a C-contiguous float64 array")
     return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def stream_probe(kind, src=None, dst=None, reps=10, stream=None):
+    """Mean milliseconds per pass of the HBM stream probe (fame_stream_probe): kind "read" (src),
+    "write" (dst) or "copy" (src -> dst); src / dst are contiguous device tensors of equal byte
+    size, a multiple of 16 bytes."""
+    lib = load_library()
+    k = {"read": 0, "write": 1, "copy": 2}[kind]
+    buf = src if src is not None else dst
+    nbytes = buf.numel() * buf.element_size()
+    ms = ctypes.c_double()
+    ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    _check(lib.fame_stream_probe(k, ptr(src), ptr(dst), nbytes, int(reps), stream, ctypes.byref(ms)))
+    return ms.value
 
 
 class Plan:

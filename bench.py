@@ -77,28 +77,120 @@ def cpu_baseline(n, seconds):
                       f"{cnt} calls in {dt:.1f} s; scipy.fft workers={fo.FFT_WORKERS}, CSR SpMV single-threaded"}
 
 
-def hbm_copy_rate(torch, nbytes=1 << 31, reps=10, trials=3):
-    """Device-to-device copy of ``nbytes`` (2 GiB: far beyond the 256 MB Infinity Cache),
-    timed with HIP events, best of ``trials``; GB/s counting the read and the write."""
-    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    ms = min(_time_op(torch, lambda: dst.copy_(src), reps, warm=3) for _ in range(trials))
-    del src, dst
-    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+def achievable_rates(pkg, torch, nbytes=1 << 32, reps=5, trials=3):
+    """The memory system's streaming ceiling: the library's own read-only, write-only and copy
+    kernels (fame_stream_probe, 16-byte grid-stride loops) over 4 GiB buffers, far beyond the
+    256 MB Infinity Cache; best of ``trials``; GB/s counting every byte read and written."""
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    out = {}
+    for kind, moved in (("read", nbytes), ("write", nbytes), ("copy", 2 * nbytes)):
+        ms = min(pkg.stream_probe(kind, a if kind != "write" else None, b if kind != "read" else None, reps)
+                 for _ in range(trials))
+        out[kind] = moved / (ms * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return out
 
 
-def read_traffic(path, items_per_launch):
-    """HBM bytes per launch from a profile summary, if it was measured at this launch size."""
-    if not os.path.exists(path):
+TRAFFIC_KERNELS = {"zpass_mix_in": "k_zfast", "xypass_inv": "k_xyfast", "material": "k_material_z",
+                   "xypass_fwd": "k_xyfast", "zpass_mix_out": "k_zfast"}
+
+
+def traffic_child(pkg, torch, args):
+    """Run under rocprofv3 --pmc by measure_traffic: two batched fncB calls at the configuration
+    the parent chose, then the 1 GiB copy probe whose known byte count calibrates the counters."""
+    S = pkg.structures
+    n, nk = args.n, args.nk
+    dims = (n, n, n)
+    ks = S.sc_band_path()
+    idx = np.linspace(0, len(ks) - 1, nk).round().astype(int) if nk != len(ks) else np.arange(len(ks))
+    thetas = np.stack([S.theta_of_k(ks[i], S.SC_LATTICE) for i in idx])
+    pb = pkg.Plan(dims, thetas, S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), S.woodpile_sc_ids(dims))
+    pb.set_chunk(args.chunk)
+    if args.variant:
+        pb.set_variant(*args.variant)
+    if args.mtile:
+        pb.set_material_tile(*args.mtile)
+    rng = np.random.default_rng(1234)
+    x = torch.as_tensor(rng.standard_normal((nk, 4 * n ** 3)) + 1j * rng.standard_normal((nk, 4 * n ** 3))).cuda()
+    for _ in range(2):
+        pb.fncB(x)
+    torch.cuda.synchronize()
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    pkg.stream_probe("copy", a, b, reps=1)
+    torch.cuda.synchronize()
+    pb.close()
+    print("traffic child done", flush=True)
+
+
+def _pmc_per_dispatch(db, counter):
+    """{kernel name: [(grid_z, value per dispatch), ...]} from a rocprofv3 --pmc database."""
+    import sqlite3
+
+    c = sqlite3.connect(db)
+    cols = [r[1] for r in c.execute("pragma table_info(counters_collection)")]
+    need = {"kernel_name", "counter_name", "value"}
+    if not need <= set(cols):
+        raise RuntimeError(f"unexpected counters_collection columns {cols}")
+    gz = "grid_size_z" if "grid_size_z" in cols else ("grid_z" if "grid_z" in cols else None)
+    disp = "dispatch_id" if "dispatch_id" in cols else ("id" if "id" in cols else None)
+    sel = f"select kernel_name, {gz or 1}, {disp or 'rowid'}, value from counters_collection where counter_name = ?"
+    per = {}
+    for name, z, d, v in c.execute(sel, (counter,)):
+        per.setdefault((name, z, d), 0.0)
+        per[(name, z, d)] += float(v)   # counters may come per XCD / instance: sum per dispatch
+    out = {}
+    for (name, z, d), v in per.items():
+        out.setdefault(name, []).append((z, v))
+    return out
+
+
+def measure_traffic(args, kernel, per_launch, chunk, variant, mtile, timeout=240):
+    """HBM bytes per launch of ``kernel`` from two rocprofv3 --pmc runs of ``traffic_child``
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+    for the 1 GiB copy probe in the same run (read 1 GiB, write 1 GiB).  Returns a dict, or None
+    when rocprofv3 is not available or a run fails (the bench then reports traffic null)."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
         return None
-    vals = {}
-    for line in open(path):
-        if ":" in line and not line.startswith("#"):
-            k, v = line.split(":", 1)
-            vals[k.strip()] = float(v)
-    if vals.get("items_per_launch") != items_per_launch or "traffic_bytes_per_launch" not in vals:
+    tmp = tempfile.mkdtemp(prefix="fame_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+    cmd_tail = [sys.executable, os.path.abspath(__file__), "--traffic-child", "--n", str(args.n), "--nk", str(args.nk),
+                "--chunk", str(chunk), "--variant", *map(str, variant), "--mtile", *map(str, mtile)]
+    res = {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            r = subprocess.run([prof, "--pmc", counter, "-d", d, "-o", "run", "--"] + cmd_tail,
+                               capture_output=True, text=True, timeout=timeout)
+            dbs = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
+            if r.returncode != 0 or not dbs:
+                log(f"bench: rocprofv3 --pmc {counter} failed (rc {r.returncode}): {r.stderr[-300:]}")
+                return None
+            per = _pmc_per_dispatch(dbs[0], counter)
+            cal = [v for name, vs in per.items() if "k_probe_copy" in name for _, v in vs]
+            hot = [v for name, vs in per.items() if kernel in name for z, v in vs if z == per_launch]
+            if not cal or not hot:
+                log(f"bench: {counter}: no dispatches of {kernel} (grid z {per_launch}) or of the copy probe")
+                return None
+            scale = float(1 << 30) / (np.mean(cal) * 1024.0)      # bytes per counted KiB
+            res[counter] = {"kib_per_dispatch": float(np.mean(hot)), "copy_probe_kib": float(np.mean(cal)),
+                            "scale": scale, "bytes_per_launch": float(np.mean(hot)) * 1024.0 * scale,
+                            "dispatches": len(hot)}
+    except (subprocess.TimeoutExpired, OSError, RuntimeError) as e:
+        log(f"bench: traffic measurement skipped: {e!r}")
         return None
-    return vals["traffic_bytes_per_launch"]
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    res["bytes_per_launch"] = res["FETCH_SIZE"]["bytes_per_launch"] + res["WRITE_SIZE"]["bytes_per_launch"]
+    return res
 
 
 def _time_op(torch, fn, reps, warm=2):
@@ -658,6 +750,9 @@ def main():
                     help="fncB matvec rates at n = 24, 48, 60, 72 (batched and plain) with the CPU oracle's")
     ap.add_argument("--chunk", type=int, default=-1,
                     help="k-points per fncB chunk (-1: autotune on the plan, 0: whole batch)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM traffic")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -670,6 +765,10 @@ def main():
     import __graft_entry__ as ge
 
     pkg = ge.load_package()
+    if args.traffic_child:
+        torch.cuda.set_device(0)
+        traffic_child(pkg, torch, args)
+        return
     if args.sweep:
         torch.cuda.set_device(0)
         run_sweep(pkg, torch, args)
@@ -723,6 +822,10 @@ def main():
             pb.set_material_tile(*args.mtile)
         log(f"bench: fixed chunk = {chunk}, xy/z launch variants = {pb.variant()}, material tile = {pb.material_tile()}")
     x1, y1 = xb[0], yb[0]
+    # the plain path at its best: the one-k plan autotuned the same way (launch shapes, tile)
+    _, tune1_ms = p1.autotune()
+    log(f"bench: plain plan autotuned: xy/z variants = {p1.variant()}, material tile = {p1.material_tile()} "
+        f"({tune1_ms:.4f} ms per fncB)")
 
     def run_plain():
         rc = lib.fame_fncB(p1.h, ctypes.c_void_p(x1.data_ptr()), ctypes.c_void_p(y1.data_ptr()))
@@ -773,14 +876,28 @@ def main():
     launch_bytes = STAGE_BYTES_PER_POINT[dom] * N * per_launch
     launch_ms = stage_avg[dom] * per_launch / nk  # a full-size launch (the last one may be shorter)
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-    traffic = read_traffic(os.path.join(ROOT, "profiles", f"traffic_{dom}_n{n}.txt"), per_launch)
-    copy_gbs = hbm_copy_rate(torch)
-    log(f"bench: device copy {copy_gbs:.0f} GB/s (2 GiB, read + write)")
+    ceil = achievable_rates(pkg, torch)
+    log("bench: stream probes over 4 GiB: " + ", ".join(f"{k} {v:.0f} GB/s" for k, v in ceil.items()))
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "algo_bytes_per_launch": launch_bytes, "items_per_launch": per_launch,
                 "launches_per_step": launches, "avg_launch_us": launch_ms * 1e3,
-                "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs}
+                "achievable_GBs": ceil, "frac_of_achievable_copy": achieved / ceil["copy"]}
+    in_profiler = any(k.startswith("ROCPROF") for k in os.environ)
+    if not args.no_traffic and not in_profiler and dom in ("material",):
+        log("bench: HBM traffic of the dominant kernel (two rocprofv3 --pmc child runs)")
+        tr = measure_traffic(args, TRAFFIC_KERNELS[dom], per_launch, chunk, list(pb.variant()),
+                             list(pb.material_tile()))
+        if tr is not None:
+            roofline["traffic"] = tr["bytes_per_launch"]
+            roofline["traffic_over_algorithmic"] = tr["bytes_per_launch"] / launch_bytes
+            roofline["traffic_detail"] = {
+                "counters": {c: tr[c] for c in ("FETCH_SIZE", "WRITE_SIZE")},
+                "note": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, TCC to fabric, so Infinity-"
+                        "Cache hits count too), each scaled by its reading for the 1 GiB copy probe in the same run; "
+                        "measured in this run at this configuration"}
+            log(f"bench: traffic {tr['bytes_per_launch'] / 1e6:.1f} MB per launch "
+                f"({roofline['traffic_over_algorithmic']:.2f}x algorithmic)")
     log("bench: stage ms " + ", ".join(f"{k}={v:.4f}" for k, v in stage_avg.items()))
     fncb_gbs = FNCB_ALGO_BYTES_PER_POINT * N * value / 1e9
 

@@ -223,6 +223,14 @@ int fame_plan_set_vertex_material(fame_plan* plan, const double* eps, const doub
  * For fncB x = w fncA x, dw/dp = grad / (x^H fncA x).  Needs the per-vertex material. */
 int fame_material_gradient(fame_plan* plan, int nvec, const void* ev, void* grad);
 
+/* HBM stream probe (no reference counterpart; a measurement aid): kind 0 reads `bytes` of src,
+ * 1 writes `bytes` of dst, 2 copies src to dst, with plain 16-byte grid-stride kernels, `reps`
+ * times after one warm-up on the given hipStream_t (NULL: the default stream); ms_per_rep gets
+ * the mean HIP-event time per pass.  With buffers far beyond the 256 MB Infinity Cache the rate
+ * is the memory system's streaming ceiling.  bytes: a positive multiple of 16. */
+int fame_stream_probe(int kind, const void* src, void* dst, long long bytes, int reps, void* stream,
+                      double* ms_per_rep);
+
 /* ---------------------------------------------------------------------------
  * FCC skewed-grid operators (the reference's light_locked_Huang/ FCC path).
  *

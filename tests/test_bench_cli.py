@@ -1,6 +1,5 @@
 """bench.py's command-line contract on the CPU: more than one GPU is refused with a message,
-the options parse, and the roofline's traffic figure is only taken from a profile summary
-measured at the same launch size."""
+and the options parse."""
 import json
 import os
 import subprocess
@@ -21,16 +20,6 @@ def test_help_lists_configs():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0
-    for opt in ("--config", "--chunk", "--variant", "--mtile", "--solve", "--concurrent", "--no-cpu", "--sweep"):
+    for opt in ("--config", "--chunk", "--variant", "--mtile", "--solve", "--concurrent", "--no-cpu", "--sweep",
+                "--no-traffic"):
         assert opt in r.stdout
-
-
-def test_read_traffic_matches_launch_size(tmp_path):
-    sys.path.insert(0, ROOT)
-    import bench
-
-    f = tmp_path / "traffic.txt"
-    f.write_text("# material\nitems_per_launch: 6\ntraffic_bytes_per_launch: 292490902\n")
-    assert bench.read_traffic(str(f), 6) == 292490902.0
-    assert bench.read_traffic(str(f), 40) is None          # measured at another launch size
-    assert bench.read_traffic(str(tmp_path / "missing.txt"), 6) is None
