@@ -1329,3 +1329,4 @@ int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit,
 #include "fame_fcc.inc"
 #include "fame_vertex.inc"
 #include "fame_probe.inc"
+#include "fame_noalloc.inc"

@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         "fame_minres": ([vp, vp, vp, d, d, i, pi_, pd], i),
         "fame_block_dot": ([vp, vp, i, vp, ctypes.c_longlong, vp], i),
         "fame_block_gemv": ([vp, vp, i, ctypes.c_longlong, vp, d, vp], i),
+        "fame_block_dot_noalloc": ([vp, vp, i, vp, ctypes.c_longlong, vp], i),
         "fame_mgs": ([vp, vp, i, ctypes.c_longlong, vp, vp], i),
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
@@ -133,7 +134,7 @@ def exported_symbols():
         "fame_version", "fame_last_error", "fame_plan_create", "fame_plan_set_theta", "fame_plan_destroy",
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
-        "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs",
+        "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs", "fame_block_dot_noalloc",
         "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused", "fame_plan_set_cg_graph",
         "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",

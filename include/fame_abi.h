@@ -189,6 +189,13 @@ int fame_minres(fame_plan* plan, const void* in, void* out, double sigma, double
 int fame_block_dot(fame_plan* plan, const void* V, int m, const void* w, long long len, void* out);
 int fame_block_gemv(fame_plan* plan, const void* V, int m, long long len, const void* y, double beta, void* out);
 
+/* fame_block_dot without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate (HIP graph capture, tight serving loops).  The per-work-group partial sums
+ * that fame_block_dot allocates per call live in a file-scope device array of 2^18 complex128
+ * (4 MiB per GPU, held until the process ends), so m * ceil(len / 1024) <= 262144, else
+ * FAME_ERR_ARG before any device work.  Calls that overlap on one GPU share that array and race. */
+int fame_block_dot_noalloc(fame_plan* plan, const void* V, int m, const void* w, long long len, void* out);
+
 /* Modified Gram-Schmidt of t (len) against the m columns of V (column-major, len each), in
  * column order, on the device: c_j = V_j^H t; t <- t - c_j V_j for j = 0..m-1
  * (light_locked_Huang/GEP_AB_Herm_JDSIRA_Locking4.m:326-330, the orthogonalisation of the
