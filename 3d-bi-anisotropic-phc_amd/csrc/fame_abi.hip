@@ -92,6 +92,8 @@ struct fame_plan {
   void* d_mrst = nullptr;     // MINRES per-item state [nb]
   double2* d_mgs = nullptr;   // fame_mgs coefficients when the caller passes none
   int mgs_cap = 0;
+  double2* d_bdp = nullptr;   // fame_block_dot partial sums, grown to the largest m * work-groups seen
+  size_t bdp_cap = 0;
   bool cg_fused = true;       // CG vector updates fused into fncB (specialised z kernels only)
   int cg_seq = 0;             // > 0: fused CG runs chunks of this many items to convergence one after another
   int mt[3] = {0, 0, 0};      // material stencil tile TX, TY and slab TZ (0: heuristic)
@@ -1022,7 +1024,7 @@ int fame_plan_destroy(fame_plan* P) {
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
-                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs};
+                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs, P->d_bdp};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
@@ -1330,3 +1332,4 @@ int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit,
 #include "fame_vertex.inc"
 #include "fame_probe.inc"
 #include "fame_noalloc.inc"
+#include "fame_voxel.inc"

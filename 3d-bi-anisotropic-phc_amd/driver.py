@@ -13,14 +13,17 @@ import numpy as np
 def band_structure_lanczos(pkg, dims, thetas, table, ids, nev=10, tol=1e-10, itmax=10000, p=None, batched=True,
                            restore=True, mat_path=None, delta=None, cg_tol=1e-12, cg_maxit=10000):
     """Returns a dict with Freq_mtx (nev, nk), iter_records (nk,), compute_eigs (nk,) seconds per
-    k-point (table setup through residuals, as the reference's time_eig window), and, with
-    ``restore``, the restored fields and the NFSEP / original-problem residuals."""
+    k-point (table setup through residuals, as the reference's time_eig window), not_converged
+    (the k indices whose Lanczos stopped at itmax or whose inner CG reached cg_maxit; their
+    columns hold the last Ritz values, as Lanczos.m returns at mm == maxiter, and their
+    iter_records are negated), and, with ``restore``, the restored fields and the NFSEP /
+    original-problem residuals."""
     import torch
 
     thetas = np.atleast_2d(np.asarray(thetas, float))
     nk = thetas.shape[0]
     p = 2 * nev if p is None else p
-    lz = dict(tol=tol, maxiter=itmax, p=p, cg_tol=cg_tol, cg_maxit=cg_maxit)
+    lz = dict(tol=tol, maxiter=itmax, p=p, cg_tol=cg_tol, cg_maxit=cg_maxit, raise_on_maxit=False)
     F = np.zeros((nev, nk))
     iters = np.zeros(nk, int)
     secs = np.zeros(nk)
@@ -70,7 +73,8 @@ def band_structure_lanczos(pkg, dims, thetas, table, ids, nev=10, tol=1e-10, itm
                 rn_all.append(rn[0])
                 ro_all.append(ro[0])
             plan.close()
-    out = {"Freq_mtx": F, "iter_records": iters, "compute_eigs": secs}
+    out = {"Freq_mtx": F, "iter_records": iters, "compute_eigs": secs,
+           "not_converged": np.nonzero(iters < 0)[0].tolist()}
     if restore:
         out.update(e_h=np.stack(e_h_all), res_NFSEP=np.stack(rn_all), res_ORGSEP=np.stack(ro_all))
         if mat_path:

@@ -63,6 +63,8 @@ def load_library(path: str = LIB_PATH):
         "fame_last_error": ([], ctypes.c_char_p),
         "fame_plan_create": ([ctypes.POINTER(vp), i, i, i, i, pd, pd, i, pd, ctypes.c_void_p], i),
         "fame_plan_set_theta": ([vp, pd], i),
+        "fame_plan_set_structure": ([vp, i, pd, pd], i),
+        "fame_plan_material_ids": ([vp, vp], i),
         "fame_plan_destroy": ([vp], i),
         "fame_sync": ([vp], i),
         "fame_plan_stream": ([vp], vp),
@@ -132,6 +134,7 @@ def exported_symbols():
     """Names of the C-ABI entry points this wrapper binds."""
     return [
         "fame_version", "fame_last_error", "fame_plan_create", "fame_plan_set_theta", "fame_plan_destroy",
+        "fame_plan_set_structure", "fame_plan_material_ids",
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs", "fame_block_dot_noalloc",
@@ -239,6 +242,25 @@ class Plan:
             raise ValueError("theta must be (nb, 3)")
         self.theta = th.copy()
         _check(self.lib.fame_plan_set_theta(self.h, _dptr(th)))
+
+    STRUCTURES = ("woodpile", "sphere", "tetra")
+
+    def set_structure(self, kind, size, center=(0.5, 0.5, 0.5), lattice=None, a=1.0):
+        """Per-vertex material ids computed on the device (fame_plan_set_structure): "woodpile"
+        (rod width ``size``, lattice constant ``a``), "sphere" (radius ``size``) or "tetra" (edge
+        ``size``) centred at lattice coordinates ``center``, lattice vectors as the columns of
+        ``lattice``; id 1 inside, 0 outside."""
+        k = self.STRUCTURES.index(kind)
+        params = np.ascontiguousarray([float(size), float(a), 0.0, 0.0] if k == 0 else
+                                      [float(size), *map(float, center)], dtype=np.float64)
+        lat = np.ascontiguousarray(np.eye(3) if lattice is None else np.asarray(lattice, float)).reshape(-1)
+        _check(self.lib.fame_plan_set_structure(self.h, k, _dptr(params), _dptr(lat)))
+
+    def material_ids(self):
+        """The per-vertex material ids the stencil reads (device uint8 tensor, N)."""
+        out = self.torch.empty(self.N, dtype=self.torch.uint8, device=self.device)
+        _check(self.lib.fame_plan_material_ids(self.h, ctypes.c_void_p(out.data_ptr())))
+        return out
 
     def sync(self):
         _check(self.lib.fame_sync(self.h))
@@ -405,8 +427,12 @@ class Plan:
                                  None if c is None else ctypes.c_void_p(c.data_ptr())))
         return c
 
-    def Lanczos(self, k, tol=1e-10, maxiter=10000, p=None, v0=None, cg_tol=1e-12, cg_maxit=10000):
-        """Thick-restart Lanczos (Lanczos.m:1-156); returns (ev [nb,k,4N], ew [nb,k], iters [nb])."""
+    def Lanczos(self, k, tol=1e-10, maxiter=10000, p=None, v0=None, cg_tol=1e-12, cg_maxit=10000,
+                raise_on_maxit=True):
+        """Thick-restart Lanczos (Lanczos.m:1-156); returns (ev [nb,k,4N], ew [nb,k], iters [nb]).
+
+        iters[b] < 0 marks a k-point that did not converge (its ev/ew are the last Ritz pairs);
+        with raise_on_maxit (default) any such k-point raises NotConverged instead."""
         p = 2 * k if p is None else p
         if v0 is None:
             v0 = self.torch.ones((self.nb, 4 * self.N), dtype=self.torch.complex128, device=self.device)
@@ -417,7 +443,8 @@ class Plan:
         rc = self.lib.fame_lanczos(self.h, int(k), float(tol), int(maxiter), int(p), ctypes.c_void_p(v.data_ptr()),
                                    float(cg_tol), int(cg_maxit), ctypes.c_void_p(ev.data_ptr()),
                                    ew.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), iters)
-        _check(rc)
+        if rc != FAME_ERR_NOT_CONVERGED or raise_on_maxit:
+            _check(rc)
         return ev, ew, np.array(iters[:])
 
     def Eigen_Restoration_Chiral_no_inv(self, ev):

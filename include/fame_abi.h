@@ -60,6 +60,20 @@ const char* fame_last_error(void);
 int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const double* theta, const double* delta,
                      int n_mat, const double* mat_table, const unsigned char* mat_id);
 
+/* Material ids computed on the device (replaces the host mat_id of fame_plan_create and the
+ * host restatement of subroutines/Material_Locate_Isofunction.m:1-148): id 1 inside the
+ * structure, 0 outside, at the vertices (i1/n1, i2/n2, i3/n3) (Grid_Generate_Mult_Point.m:7-17).
+ *   kind 0: SC woodpile, params {rod width, a}: four layers of height a/4 with rods along x
+ *           at y = a/4, along y at x = a/4, along x at y = 3a/4, along y at x = 3a/4
+ *   kind 1: sphere, params {radius, c1, c2, c3} (centre in lattice coordinates)
+ *   kind 2: regular tetrahedron, params {edge, c1, c2, c3}, vertices c + edge/sqrt(8) (+-1, +-1,
+ *           +-1) with an even number of minus signs
+ * Spheres and tetrahedra use `lattice` (row-major 3x3, lattice vectors as columns) and the
+ * minimum image over the 27 neighbouring cells.  The plan's table needs n_mat >= 2. */
+int fame_plan_set_structure(fame_plan* plan, int kind, const double* params, const double* lattice);
+/* Copy the plan's per-vertex material ids (N bytes) to device memory `dst`. */
+int fame_plan_material_ids(fame_plan* plan, void* dst);
+
 /* Change the Bloch phases of all nb k-points (theta [nb][3]); grid and material stay. */
 int fame_plan_set_theta(fame_plan* plan, const double* theta);
 
@@ -167,8 +181,12 @@ int fame_fncinvB(fame_plan* plan, const void* in, void* out, double tol, int max
  *   v0       [nb][4N] device start vectors (B-normalised inside, Lanczos.m:8)
  *   ev       [nb][k][4N] device output (eigenvectors, column-major per item)
  *   ew       [nb][k] host output: eigenvalues d of B^{-1}A (omega = 1/d)
- *   iters    [nb] host output: Lanczos steps taken (may be NULL)
- * Returns FAME_ERR_NOT_CONVERGED when maxiter restarts did not converge all items. */
+ *   iters    [nb] host output: Lanczos steps taken (may be NULL); negated (-steps) for an item
+ *            that did not converge (maxiter restarts reached, or an inner CG solve of a step it
+ *            took part in reached cg_maxit)
+ * Returns FAME_ERR_NOT_CONVERGED when any item did not converge; ev and ew are still written for
+ * every item (the converged ones are final, the others hold the last Ritz pairs, as Lanczos.m
+ * returns its iterates at mm == maxiter). */
 int fame_lanczos(fame_plan* plan, int k, double tol, int maxiter, int p, const void* v0, double cg_tol, int cg_maxit,
                  void* ev, double* ew, int* iters);
 
@@ -191,7 +209,8 @@ int fame_block_gemv(fame_plan* plan, const void* V, int m, long long len, const 
 
 /* fame_block_dot without any allocation.  No counterpart in the reference: it is for callers
  * that cannot allocate (HIP graph capture, tight serving loops).  The per-work-group partial sums
- * that fame_block_dot allocates per call live in a file-scope device array of 2^18 complex128
+ * that fame_block_dot allocates (a plan-owned buffer, on first use and whenever a larger
+ * projection comes) live in a file-scope device array of 2^18 complex128
  * (4 MiB per GPU, held until the process ends), so m * ceil(len / 1024) <= 262144, else
  * FAME_ERR_ARG before any device work.  Calls that overlap on one GPU share that array and race. */
 int fame_block_dot_noalloc(fame_plan* plan, const void* V, int m, const void* w, long long len, void* out);

@@ -60,3 +60,23 @@ def test_band_structure_driver_plain_and_batched(pkg, torch_mod, tmp_path):
     assert rb["res_ORGSEP"].max() < 1e-6
     m = scipy.io.loadmat(str(path))
     assert np.allclose(m["Freq_mtx"], rb["Freq_mtx"])
+    assert rb["not_converged"] == [] and rp["not_converged"] == []
+
+
+def test_lanczos_reports_unconverged_items(pkg, torch_mod):
+    """maxiter = 1 restart with a tight tolerance leaves the k-points unconverged: the call raises
+    NotConverged by default; with raise_on_maxit=False it returns every item's last Ritz pairs
+    (finite, ev written) with iters negated, and a full solve afterwards reports positive steps."""
+    print("Lanczos per-item convergence flags, n=6, 2 k", flush=True)
+    dims = (6, 6, 6)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE) for k in ([0.5, 0.3, 0.1], [0.2, 0.4, 0.6])])
+    plan = pkg.Plan(dims, thetas, S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), S.woodpile_sc_ids(dims))
+    with pytest.raises(pkg.NotConverged):
+        plan.Lanczos(4, tol=1e-14, maxiter=1)
+    ev, ew, it = plan.Lanczos(4, tol=1e-14, maxiter=1, raise_on_maxit=False)
+    assert np.all(it < 0) and np.all(np.isfinite(ew)) and np.all(ew != 0)
+    assert bool(torch_mod.isfinite(ev).all()) and float(ev.abs().max()) > 0
+    ev, ew, it = plan.Lanczos(4, tol=1e-10, maxiter=10000, raise_on_maxit=False)
+    assert np.all(it > 0)
+    plan.close()
