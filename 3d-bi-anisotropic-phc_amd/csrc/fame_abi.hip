@@ -94,6 +94,8 @@ struct fame_plan {
   int mgs_cap = 0;
   double2* d_bdp = nullptr;   // fame_block_dot partial sums, grown to the largest m * work-groups seen
   size_t bdp_cap = 0;
+  double2* d_jdy = nullptr;   // JD-SIRA step coefficients / projection columns
+  size_t jdy_cap = 0;
   bool cg_fused = true;       // CG vector updates fused into fncB (specialised z kernels only)
   int cg_seq = 0;             // > 0: fused CG runs chunks of this many items to convergence one after another
   int mt[3] = {0, 0, 0};      // material stencil tile TX, TY and slab TZ (0: heuristic)
@@ -574,16 +576,20 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
   // converged items of a fused CG are skipped by every pass (their state is frozen)
   const CgState* cgst = pcf ? wcf.st : nullptr;
-  RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6, cgst));
+  // the xy passes run in place (each work-group loads its whole plane before its first barrier
+  // and stores it after its last): a chunk's fields then occupy two buffers that are each read
+  // and rewritten on the same lines, which keeps more of them in the Infinity Cache between
+  // passes (1.2 % faster fncB at n = 60, 40 k, same results bit for bit)
+  RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w1, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
-  RET_IF(material(P, P->d_w2, 6 * N, P->d_w1, 6 * N, pcf ? wcf.part : nullptr, pcf ? wcf.nparts : 0, cgst));
+  RET_IF(material(P, P->d_w1, 6 * N, P->d_w2, 6 * N, pcf ? wcf.part : nullptr, pcf ? wcf.nparts : 0, cgst));
   if (pcf) {
     hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, material_parts(P, nbc), wcf.nparts,
                        wcf.st);
     CHECK_LAUNCH();
   }
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
-  RET_IF(xypass(P, -1, 0, P->d_w1, 6 * N, P->d_w2, 6 * N, 6, cgst));
+  RET_IF(xypass(P, -1, 0, P->d_w2, 6 * N, P->d_w2, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
   RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, pcf));
   if (pr) {
@@ -1024,7 +1030,7 @@ int fame_plan_destroy(fame_plan* P) {
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
-                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs, P->d_bdp};
+                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs, P->d_bdp, P->d_jdy};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
@@ -1165,6 +1171,14 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
       }
     std::sort(tiles.begin(), tiles.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
     if (tiles.size() > 6) tiles.resize(6);
+    // wide tiles: longer contiguous runs per row (DRAM page locality) at a larger halo share
+    for (int tx : {30, 40, 60})
+      for (int ty = 1; ty <= 30; ++ty)
+        if (tx <= P->n1 && ty <= P->n2 && (tx + 2) * (ty + 2) <= kMzThreads && (tx + 2) * (ty + 3) > kMzThreads) {
+          bool dup = false;
+          for (const Cand& c : tiles) dup |= (c.tx == tx && c.ty == ty);
+          if (!dup) tiles.push_back({0.0, tx, ty});
+        }
     int hx, hy, hz;
     material_tile(P->n1, P->n2, P->n3, (bestc > 0 ? bestc : P->nb), hx, hy, hz);
     int btx = 0, bty = 0, btz = 0;
@@ -1333,3 +1347,4 @@ int fame_fncinvB(fame_plan* P, const void* in, void* out, double tol, int maxit,
 #include "fame_probe.inc"
 #include "fame_noalloc.inc"
 #include "fame_voxel.inc"
+#include "fame_jd.inc"

@@ -86,6 +86,8 @@ def load_library(path: str = LIB_PATH):
         "fame_block_gemv": ([vp, vp, i, ctypes.c_longlong, vp, d, vp], i),
         "fame_block_dot_noalloc": ([vp, vp, i, vp, ctypes.c_longlong, vp], i),
         "fame_mgs": ([vp, vp, i, ctypes.c_longlong, vp, vp], i),
+        "fame_jd_ritz_residual": ([vp, vp, i, ctypes.c_longlong, pd, d, d, vp, vp, pd], i),
+        "fame_jd_expand": ([vp, vp, i, ctypes.c_longlong, vp, d, d, i, pi_, pd], i),
         "fame_plan_profile": ([vp, i], i),
         "fame_plan_set_generic": ([vp, i], i),
         "fame_plan_set_cg_fused": ([vp, i], i),
@@ -137,7 +139,7 @@ def exported_symbols():
         "fame_plan_set_structure", "fame_plan_material_ids",
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
-        "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs", "fame_block_dot_noalloc",
+        "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs", "fame_block_dot_noalloc", "fame_jd_ritz_residual", "fame_jd_expand",
         "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused", "fame_plan_set_cg_graph",
         "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
@@ -426,6 +428,35 @@ class Plan:
         _check(self.lib.fame_mgs(self.h, ctypes.c_void_p(V.data_ptr()), int(m), int(L), ctypes.c_void_p(t.data_ptr()),
                                  None if c is None else ctypes.c_void_p(c.data_ptr())))
         return c
+
+    def jd_ritz_residual(self, V, y, theta):
+        """u = V^T y / ||.||, r = fncB u - theta fncA u and ||r|| for V (m, len) device rows and host
+        coefficients y (m,) (Locking4.m:170-178); returns (u, r, ||r||)."""
+        m, L = V.shape
+        yv = np.ascontiguousarray(np.asarray(y, complex).reshape(m)).view(np.float64)
+        u = self.torch.empty(L, dtype=self.torch.complex128, device=self.device)
+        r = self.torch.empty_like(u)
+        rn = ctypes.c_double()
+        th = complex(theta)
+        _check(self.lib.fame_jd_ritz_residual(self.h, ctypes.c_void_p(V.data_ptr()), int(m), int(L), _dptr(yv),
+                                              th.real, th.imag, ctypes.c_void_p(u.data_ptr()),
+                                              ctypes.c_void_p(r.data_ptr()), ctypes.byref(rn)))
+        return u, r, rn.value
+
+    def jd_expand(self, V, m, r, sigma, tol, maxit):
+        """MINRES correction of r, orthogonalised and normalised into row m of V (a contiguous
+        (rows, len) device tensor), and the new column of V^H fncB V and V^H fncA V over rows
+        0..m (Locking4.m:311-341); returns (MINRES steps, column of V^H A V, column of V^H B V)."""
+        L = V.shape[1]
+        if not V.is_contiguous() or m + 1 > V.shape[0]:
+            raise ValueError("jd_expand: V must be contiguous with room for row m")
+        proj = np.zeros(2 * (m + 1), complex)
+        its = ctypes.c_int()
+        pv = proj.view(np.float64)
+        _check(self.lib.fame_jd_expand(self.h, ctypes.c_void_p(V.data_ptr()), int(m), int(L),
+                                       ctypes.c_void_p(r.data_ptr()), float(sigma), float(tol), int(maxit),
+                                       ctypes.byref(its), _dptr(pv)))
+        return its.value, proj[:m + 1], proj[m + 1:]
 
     def Lanczos(self, k, tol=1e-10, maxiter=10000, p=None, v0=None, cg_tol=1e-12, cg_maxit=10000,
                 raise_on_maxit=True):

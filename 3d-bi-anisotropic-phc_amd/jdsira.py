@@ -79,13 +79,6 @@ def _norm_dev(plan, x):
     return plan.block_dot(x.reshape(1, -1), x)[0].abs().sqrt()
 
 
-def _proj_cols(plan, V, AV, BV):
-    """V^H AV and V^H BV, fetched to the host in one transfer."""
-    t = plan.torch
-    both = t.stack([plan.block_dot(V, AV), plan.block_dot(V, BV)]).cpu().numpy()
-    return both[0], both[1]
-
-
 def choose_ritz_pair(iteno, ew, ew_order, target, VRR, vec_r_tol, choose_type, rppd, target_org, choose_type_org,
                      zflag_ew, flag_no_ew):
     """Select Ritz pairs (Choose_Ritz_Pair_Unsymm4.m:1-412); returns (ewno, ritz_ew, ritz_ev)."""
@@ -285,10 +278,8 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
             ewno, ritz_ew, ritz_ev = choose_ritz_pair(iteno, ew, ew_order, target_ritz_ew, VRR, vec_r_tol,
                                                       choose_type, rppd, target, choose_type_org, st.zflag_ew,
                                                       st.flag_no_ew)
-            ev_crnt = plan.block_gemv(V[:iteno], t.as_tensor(ritz_ev[:iteno, 0]))
-            ev_crnt = ev_crnt / _norm_dev(plan, ev_crnt)
-            rsdl_vec = A(ev_crnt) - ritz_ew[0] * B(ev_crnt)
-            rsdl = _norm(plan, rsdl_vec)
+            # u = V y / ||V y||, r = A u - theta B u, ||r|| (:170-178): one library call
+            ev_crnt, rsdl_vec, rsdl = plan.jd_ritz_residual(V[:iteno], ritz_ev[:iteno, 0], ritz_ew[0])
             st.log.append((ew_order, EPIteNo, complex(ritz_ew[0]), rsdl))
             if 10.0 * vec_r_tol >= rsdl > vec_r_tol:
                 no_ew_near_tol += 1
@@ -316,15 +307,12 @@ def _locking(plan, A, B, EV, ew_order, init_vec, proj_dim, target, choose_type, 
                             iter_flag = 1
                     # SIRA correction: inner MINRES on A (or A - sigma B), :311-312
                     sig = 0.0 if sigma_inner is None else float(sigma_inner)
-                    tv, its, _ = plan.minres(rsdl_vec, sigma=sig, tol=linear_system_tol, maxit=pcg_iter_number)
-                    st.total_inner_iter += int(its[0])
-                    nrm = _norm_dev(plan, tv)  # a tiny correction is rescaled first (:318-320)
-                    tv = (tv * t.where(nrm < 1.0e-9, 1.0 / nrm, t.ones_like(nrm))).contiguous()
-                    plan.mgs(V[:iteno], tv)  # orthogonalise against V, MGS in column order (:326-330)
-                    V[iteno] = tv / _norm_dev(plan, tv)
-                    # update V^H A V and V^H B V (:335-341)
-                    Mm[:iteno + 1, iteno], Lm[:iteno + 1, iteno] = _proj_cols(plan, V[:iteno + 1], A(V[iteno]),
-                                                                              B(V[iteno]))
+                    # MINRES correction, rescaled when tiny (:318-320), orthogonalised against V by
+                    # MGS in column order (:326-330) and normalised into V[iteno], then the new column
+                    # of V^H A V and V^H B V (:335-341): one library call
+                    its, mcol, lcol = plan.jd_expand(V, iteno, rsdl_vec, sig, linear_system_tol, pcg_iter_number)
+                    st.total_inner_iter += int(its)
+                    Mm[:iteno + 1, iteno], Lm[:iteno + 1, iteno] = mcol, lcol
                     Mm[iteno, :iteno] = np.conj(Mm[:iteno, iteno])
                     Lm[iteno, :iteno] = np.conj(Lm[:iteno, iteno])
             iteno += 1

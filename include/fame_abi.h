@@ -223,6 +223,20 @@ int fame_block_dot_noalloc(fame_plan* plan, const void* V, int m, const void* w,
  * plan's stream, no host synchronisation. */
 int fame_mgs(fame_plan* plan, const void* V, int m, long long len, void* t, void* coef);
 
+/* One JD-SIRA outer step in two calls (GEP_AB_Herm_JDSIRA_Locking4.m, A = fncB, B = fncA,
+ * one k-point plan, len = 4N, V: device columns of len, column-major):
+ *   fame_jd_ritz_residual: u = V y / ||V y|| for the m host coefficients y (complex), and
+ *     r = A u - theta B u with rnorm = ||r|| on the host (Locking4.m:170-178); u, r: device, len.
+ *   fame_jd_expand: t = MINRES on (A - sigma B) t = r (tol, maxit; fame_minres), rescaled when
+ *     ||t|| < 1e-9, orthogonalised against the m columns of V by MGS in column order and
+ *     normalised into column m of V (Locking4.m:311-330); proj (host, 2(m+1) complex) receives
+ *     V^H A t and V^H B t over the m + 1 columns (Locking4.m:335-341); iters: MINRES steps.
+ * Each call synchronises the plan's stream once, for its host result. */
+int fame_jd_ritz_residual(fame_plan* plan, const void* V, int m, long long len, const double* y, double theta_re,
+                          double theta_im, void* u, void* r, double* rnorm);
+int fame_jd_expand(fame_plan* plan, void* V, int m, long long len, const void* r, double sigma, double tol, int maxit,
+                   int* iters, double* proj);
+
 /* Eigen_Restoration_Chiral_no_inv (fun_no_inv/Eigen_Restoration_Chiral_no_inv.m:3-25):
  * for nvec NFSEP vectors per item ([nb][nvec][4N]) returns [nb][nvec][6N] =
  * [e (3N); h (3N)] with e = -i (N Qr(v_b) + J Pr(v_a)), h = -i (J^H Qr(v_b) + M Pr(v_a)). */
@@ -250,7 +264,7 @@ int fame_plan_set_vertex_material(fame_plan* plan, const double* eps, const doub
 int fame_material_gradient(fame_plan* plan, int nvec, const void* ev, void* grad);
 
 /* HBM stream probe (no reference counterpart; a measurement aid): kind 0 reads `bytes` of src,
- * 1 writes `bytes` of dst, 2 copies src to dst, with plain 16-byte grid-stride kernels, `reps`
+ * 1 writes `bytes` of dst, 2 copies src to dst, with plain 16-byte streaming kernels, `reps`
  * times after one warm-up on the given hipStream_t (NULL: the default stream); ms_per_rep gets
  * the mean HIP-event time per pass.  With buffers far beyond the 256 MB Infinity Cache the rate
  * is the memory system's streaming ceiling.  bytes: a positive multiple of 16. */

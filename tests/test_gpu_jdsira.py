@@ -178,3 +178,37 @@ def test_jdsira_woodpile_matches_dense(pkg, fo, torch_mod):
         assert abs(np.linalg.norm(u) - 1) < 1e-12
         assert np.linalg.norm(Bd @ u - res.ew[j] * (Ad @ u)) <= 1e-7
     plan.close()
+
+
+def test_fused_jd_steps_match_composed_primitives(pkg, fo, torch_mod):
+    """fame_jd_ritz_residual and fame_jd_expand (one outer JD-SIRA step in two calls) give what
+    the separate device primitives give: u = V y / ||V y||, r = fncB u - theta fncA u, ||r||;
+    MINRES, the tiny-norm rescale, MGS, normalisation and the two projection columns."""
+    say("fused JD-SIRA steps == composed primitives, n=8")
+    t = torch_mod
+    dims = (8, 8, 8)
+    plan, _ = woodpile_plan(pkg, fo, dims, THETAS[0])
+    L = 4 * plan.N
+    m = 5
+    V = t.zeros((m + 2, L), dtype=t.complex128, device="cuda")
+    V[:m] = t.as_tensor(crand(m, L), device="cuda")
+    for j in range(m):  # orthonormal columns, as JD-SIRA keeps them
+        plan.mgs(V[:j], V[j]) if j else None
+        V[j] /= V[j].norm()
+    y = crand(m)
+    theta = 1.37 + 0.0j
+    u, r, rn = plan.jd_ritz_residual(V[:m], y, theta)
+    u0 = plan.block_gemv(V[:m], t.as_tensor(y))
+    u0 = u0 / u0.norm()
+    r0 = plan.fncB(u0) - theta * plan.fncA(u0)
+    assert float((u - u0).norm()) < 1e-13 and float((r - r0).norm() / r0.norm()) < 1e-13
+    assert abs(rn - float(r0.norm())) < 1e-12 * rn
+    its, mcol, lcol = plan.jd_expand(V, m, r, 0.3, 1e-2, 500)
+    tv, its0, _ = plan.minres(r0, sigma=0.3, tol=1e-2, maxit=500)
+    tv = tv.contiguous()
+    plan.mgs(V[:m], tv)
+    tv = tv / tv.norm()
+    assert its == int(its0[0]) and float((V[m] - tv).norm()) < 1e-11
+    assert np.abs(mcol - plan.block_dot(V[:m + 1], plan.fncB(V[m])).cpu().numpy()).max() < 1e-12 * np.abs(mcol).max()
+    assert np.abs(lcol - plan.block_dot(V[:m + 1], plan.fncA(V[m])).cpu().numpy()).max() < 1e-12 * np.abs(lcol).max()
+    plan.close()

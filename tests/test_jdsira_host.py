@@ -59,6 +59,22 @@ class DensePlan:
         return self.torch.as_tensor(y), np.array([it]), np.array([relres])
 
 
+    def jd_ritz_residual(self, V, y, theta):
+        u = self.torch.as_tensor(np.asarray(y, complex)) @ V
+        u = u / self.torch.linalg.vector_norm(u)
+        r = self.fncB(u) - complex(theta) * self.fncA(u)
+        return u, r, float(self.torch.linalg.vector_norm(r))
+
+    def jd_expand(self, V, m, r, sigma, tol, maxit):
+        t, its, _ = self.minres(r, sigma=sigma, tol=tol, maxit=maxit)
+        nrm = float(self.torch.linalg.vector_norm(t))
+        if nrm < 1e-9:
+            t = t / nrm
+        self.mgs(V[:m], t)
+        V[m] = t / self.torch.linalg.vector_norm(t)
+        return int(its[0]), (V[:m + 1].conj() @ self.fncB(V[m])).numpy(), (V[:m + 1].conj() @ self.fncA(V[m])).numpy()
+
+
 def random_pencil(n, seed):
     rng = np.random.default_rng(seed)
     Q = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
