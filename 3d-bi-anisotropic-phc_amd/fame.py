@@ -108,6 +108,7 @@ def load_library(path: str = LIB_PATH):
         "fame_fcc_set_stream": ([vp, vp], i),
         "fame_fcc_sync": ([vp], i),
         "fame_fcc_set_generic": ([vp, i], i),
+        "fame_fcc_set_nfgep_variant": ([vp, i], i),
         "fame_fcc_T": ([vp, i, vp, vp], i),
         "fame_fcc_TH": ([vp, i, vp, vp], i),
         "fame_fcc_Sigma_r": ([vp, vp], i),
@@ -145,7 +146,7 @@ def exported_symbols():
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
         "fame_plan_set_vertex_material", "fame_material_gradient", "fame_stream_probe",
-        "fame_fcc_create", "fame_fcc_destroy", "fame_fcc_set_stream", "fame_fcc_sync", "fame_fcc_set_generic", "fame_fcc_T", "fame_fcc_TH",
+        "fame_fcc_create", "fame_fcc_destroy", "fame_fcc_set_stream", "fame_fcc_sync", "fame_fcc_set_generic", "fame_fcc_set_nfgep_variant", "fame_fcc_T", "fame_fcc_TH",
         "fame_fcc_Sigma_r", "fame_fcc_Pr", "fame_fcc_Qr", "fame_fcc_Prs", "fame_fcc_Qrs", "fame_fcc_nfgep_A",
         "fame_fcc_eigenvector", "fame_fcc_nfgep_A_ex", "fame_fcc_eigenvector_ex", "fame_fcc_nfgep_data",
         "fame_fcc_set_material", "fame_fcc_param_apply", "fame_fcc_gep_B",
@@ -604,6 +605,10 @@ class FccPlan:
                                             _dptr(dl)))
         self.h = h
         _check(self.lib.fame_fcc_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)))
+
+    def set_nfgep_variant(self, split):
+        """Fused NFGEP xy kernel: both planes in LDS (0) or one plane at a time (1); same results."""
+        _check(self.lib.fame_fcc_set_nfgep_variant(self.h, int(split)))
 
     def set_generic(self, generic=True):
         """Force the runtime-radix kernels (True) or allow the specialised ones (False)."""

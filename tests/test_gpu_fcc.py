@@ -304,3 +304,30 @@ def test_svd_maps_refuse_gamma_point(pkg, torch_mod):
         with pytest.raises(pkg.FameError, match="Gamma"):
             call()
     plan.close()
+
+
+@pytest.mark.parametrize("n", [8, 24, 48, 60, 64])
+@pytest.mark.parametrize("coupling", [0, 1])
+def test_nfgep_split_xy_kernel_equals_paired(pkg, torch_mod, n, coupling):
+    """The fused NFGEP with its xy kernel transforming one plane at a time (one plane of LDS)
+    gives the two-planes-in-LDS kernel's matvec and eigenvector map, and the generic path's."""
+    say(f"FCC NFGEP split xy kernel n={n} coupling={coupling}")
+    plan, _ = make(pkg, (n, n, n), K)
+    N = plan.n
+    t = torch_mod
+    dev = dict(device="cuda")
+    x = t.as_tensor(crand(32 * N), **dev)
+    xi, zeta = t.as_tensor(crand(24 * N), **dev), t.as_tensor(crand(24 * N), **dev)
+    phi = t.as_tensor(crand(24 * N) + 3, **dev)
+    outs = []
+    for split in (0, 1):
+        plan.set_nfgep_variant(split)
+        outs.append((plan.mtx_prod_vec_A_NFGEP(x, xi, zeta, phi, coupling=coupling),
+                     plan.Compute_eigenvector(x, xi, zeta, phi, coupling=coupling)))
+    plan.set_generic(True)
+    yg = plan.mtx_prod_vec_A_NFGEP(x, xi, zeta, phi, coupling=coupling)
+    eg = plan.Compute_eigenvector(x, xi, zeta, phi, coupling=coupling)
+    for y, e in outs:
+        assert float((y - yg).norm() / yg.norm()) < 1e-13
+        assert float((e - eg).norm() / eg.norm()) < 1e-13
+    plan.close()
