@@ -89,8 +89,9 @@ struct fame_plan {
   double2* d_part2 = nullptr; // fused-CG p^H q partials (material stencil or MIX_OUT pass)
   size_t part2_cap = 0;       // elements of d_part2
   double2* d_mr = nullptr;    // MINRES work vectors [6][nb][4N], allocated by the first fame_minres
-  void* d_mrst = nullptr;     // MINRES per-item state [2][nb] (the shifted iteration alternates halves)
-  double2* d_mrp = nullptr;   // MINRES partial sums of the folded kernels [3][nb][nblk_red]
+  void* d_mrst = nullptr;     // MINRES per-item state [2][nb] (the iteration alternates halves)
+  double2* d_mrp = nullptr;   // MINRES partial sums [3][nb][2048] and alpha [nb]
+  double2* d_mgp = nullptr;   // fame_mgs partial sums [2][1024]
   double2* d_mgs = nullptr;   // fame_mgs coefficients when the caller passes none
   int mgs_cap = 0;
   double2* d_bdp = nullptr;   // fame_block_dot partial sums, grown to the largest m * work-groups seen
@@ -1031,7 +1032,8 @@ int fame_plan_destroy(fame_plan* P) {
   if (P->stream) (void)hipStreamSynchronize(P->stream);
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
-                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs, P->d_bdp, P->d_jdy, P->d_mrp};
+                  P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs, P->d_bdp, P->d_jdy, P->d_mrp,
+                  P->d_mgp};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)

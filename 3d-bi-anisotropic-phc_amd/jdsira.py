@@ -81,9 +81,13 @@ def _norm_dev(plan, x):
 
 def choose_ritz_pair(iteno, ew, ew_order, target, VRR, vec_r_tol, choose_type, rppd, target_org, choose_type_org,
                      zflag_ew, flag_no_ew):
-    """Select Ritz pairs (Choose_Ritz_Pair_Unsymm4.m:1-412); returns (ewno, ritz_ew, ritz_ev)."""
+    """Select Ritz pairs (Choose_Ritz_Pair_Unsymm4.m:1-412); returns (ewno, ritz_ew, ritz_ev).
+
+    The matrices are at most a few tens wide, so the scans run over Python lists (numpy's
+    per-call overhead exceeds the work); the close-value tests come from one distance matrix."""
     ew = np.asarray(ew, complex)
-    sort_ritz = ew.copy()
+    ewl = ew[:iteno].tolist()
+    sort_ritz = list(ewl)
     sort_No = 5
     # The reference tests closeness with a loop k = -1, 0, ..., K that stops at the first
     # |a - b| <= 10^k * vec_r_tol; the thresholds grow with k, so the loop succeeds exactly
@@ -91,77 +95,84 @@ def choose_ritz_pair(iteno, ew, ew_order, target, VRR, vec_r_tol, choose_type, r
     tol_mult = 10.0 * vec_r_tol
     tol_remove = 10.0 ** sort_No * vec_r_tol
     if ew_order > 1:
-        sort_ew = np.zeros(iteno, complex)
-        sort_ew[:flag_no_ew] = zflag_ew[:flag_no_ew]
-        mult_cn_ew = np.zeros(flag_no_ew, int)
+        conv = np.asarray(zflag_ew[:flag_no_ew], complex)
+        convl = conv.tolist()
+        live = [True] * flag_no_ew            # sort_ew(i) > -1e13 in the reference
+        mult_cn_ew = [0] * flag_no_ew
+        cc = np.abs(conv[:, None] - conv[None, :]) <= tol_mult
         for i in range(flag_no_ew):  # multiplicity of converged eigenvalues (:52-78)
-            if sort_ew[i].real > -1.0e13:
+            if live[i]:
                 no_cn_ew = 1
-                for i1 in range(i + 1, flag_no_ew):
-                    if sort_ew[i1].real > -1.0e13 and abs(sort_ew[i1] - sort_ew[i]) <= tol_mult:
+                for i1 in np.nonzero(cc[i, i + 1:])[0].tolist():
+                    i1 += i + 1
+                    if live[i1]:
                         no_cn_ew += 1
-                        sort_ew[i1] = -1.0e14
+                        live[i1] = False
             else:
                 no_cn_ew = 0
             mult_cn_ew[i] = no_cn_ew
-        idx_ritz_w = np.zeros(iteno, int)  # 1-based links as in the reference (0 = first occurrence)
-        mult_ritz_w = np.zeros(iteno, int)
-        close = np.abs(sort_ritz[:, None] - sort_ritz[None, :]) <= tol_mult
+        idx_ritz_w = [0] * iteno  # 1-based links as in the reference (0 = first occurrence)
+        mult_ritz_w = [0] * iteno
+        ea = ew[:iteno]
+        close = np.abs(ea[:, None] - ea[None, :]) <= tol_mult
+        partners = {}
+        for i, j in np.argwhere(np.triu(close, 1)).tolist():
+            partners.setdefault(i, []).append(j)
         for i in range(iteno):  # multiplicity of Ritz values (:97-121)
             if idx_ritz_w[i] == 0:
                 # the inner loop only links indices it has not passed yet, so its set of
                 # links is fixed by the state before it
-                js = i + 1 + np.nonzero(close[i, i + 1:] & (idx_ritz_w[i + 1:] == 0))[0]
-                idx_ritz_w[js] = i + 1
-                mult_ritz_w[i] = 1 + js.size
+                js = [j for j in partners.get(i, ()) if idx_ritz_w[j] == 0]
+                for j in js:
+                    idx_ritz_w[j] = i + 1
+                mult_ritz_w[i] = 1 + len(js)
             else:
                 mult_ritz_w[i] = 0
+        near = np.abs(conv[:, None] - ea[None, :]) <= tol_remove
+        removed = [False] * iteno  # sort_ritz set to 1e14 (never within tol_remove again)
         for i in range(flag_no_ew):  # remove Ritz values equal to converged eigenvalues (:126-228)
-            if sort_ew[i].real > -1.0e13:
-                hits = np.nonzero((idx_ritz_w[:iteno] == 0) & (np.abs(sort_ritz[:iteno] - sort_ew[i]) <= tol_remove))[0]
-                flag = 0 if hits.size else 1
-                ind_sort = int(hits[0]) if hits.size else -1
-                if flag == 1:
-                    continue  # 'Error in Choose_Ritz_Pair' (printed by the reference, no action)
-                if mult_cn_ew[i] == mult_ritz_w[ind_sort]:
-                    sort_ritz[ind_sort] = 1.0e14
-                    sort_ritz[idx_ritz_w == ind_sort + 1] = 1.0e14
-                    idx_ritz_w[ind_sort] = -1
-                elif mult_cn_ew[i] < mult_ritz_w[ind_sort]:
-                    idx_mult = [ind_sort]
-                    ld = [np.linalg.norm(VRR[ew_order - 1:iteno, ind_sort])]
-                    for jj in range(iteno):
-                        if idx_ritz_w[jj] == ind_sort + 1:
-                            idx_mult.append(jj)
-                            ld.append(np.linalg.norm(VRR[ew_order - 1:iteno, jj]))
-                    ld = np.array(ld)
-                    for _ in range(mult_cn_ew[i]):
-                        kmin = 0
-                        tmp = ld[0]
-                        for i2 in range(1, mult_ritz_w[ind_sort]):
-                            if tmp > ld[i2]:
-                                tmp, kmin = ld[i2], i2
-                        sort_ritz[idx_mult[kmin]] = 1.0e14
-                        ld[kmin] = 1.0e14
-                    idx_ritz_w[ind_sort] = -1
-                else:
-                    sort_ritz[ind_sort] = 1.0e14
-                    sort_ritz[idx_ritz_w == ind_sort + 1] = 1.0e14
-                    idx_ritz_w[ind_sort] = -1
+            if not live[i]:
+                continue
+            ind_sort = -1
+            for r in np.nonzero(near[i])[0].tolist():
+                if idx_ritz_w[r] == 0 and not removed[r]:
+                    ind_sort = r
+                    break
+            if ind_sort < 0:
+                continue  # 'Error in Choose_Ritz_Pair' (printed by the reference, no action)
+            linked = [jj for jj in range(iteno) if idx_ritz_w[jj] == ind_sort + 1]
+            if mult_cn_ew[i] == mult_ritz_w[ind_sort] or mult_cn_ew[i] > mult_ritz_w[ind_sort]:
+                for r in [ind_sort] + linked:
+                    sort_ritz[r] = 1.0e14
+                    removed[r] = True
+            else:
+                idx_mult = [ind_sort] + linked
+                ld = [float(np.linalg.norm(VRR[ew_order - 1:iteno, r])) for r in idx_mult]
+                for _ in range(mult_cn_ew[i]):
+                    kmin = 0
+                    tmp = ld[0]
+                    for i2 in range(1, mult_ritz_w[ind_sort]):
+                        if tmp > ld[i2]:
+                            tmp, kmin = ld[i2], i2
+                    sort_ritz[idx_mult[kmin]] = 1.0e14
+                    removed[idx_mult[kmin]] = True
+                    ld[kmin] = 1.0e14
+            idx_ritz_w[ind_sort] = -1
     # candidates (:235-306)
+    tr = float(np.real(target))
     ewp_idx, dist = [], []
     for i in range(iteno):
         if abs(sort_ritz[i]) < 1.0e10:
+            e = ewl[i]
             if choose_type == "CL":
-                ok = (abs(ew[i].real) < 1e-4 and ew[i].imag > 1e-3) or (ew[i].real > np.real(target)
-                                                                          and abs(ew[i].imag) < 1e-8)
-                d = abs(ew[i].real - np.real(target))
+                ok = (abs(e.real) < 1e-4 and e.imag > 1e-3) or (e.real > tr and abs(e.imag) < 1e-8)
+                d = abs(e.real - tr)
             elif choose_type == "RGTR":
-                ok = ew[i].real > np.real(target) and abs(ew[i].imag) < 1.0e-12
-                d = abs(ew[i].real - np.real(target))
+                ok = e.real > tr and abs(e.imag) < 1.0e-12
+                d = abs(e.real - tr)
             elif choose_type == "RGTC":
-                ok = ew[i].real > np.real(target)
-                d = abs(ew[i] - target)
+                ok = e.real > tr
+                d = abs(e - target)
             else:
                 raise ValueError("In-correct Choose_Type")
             if ok:
@@ -171,34 +182,35 @@ def choose_ritz_pair(iteno, ew, ew_order, target, VRR, vec_r_tol, choose_type, r
         ewp_idx = list(range(iteno))
         dist = [abs(sort_ritz[i] - target) for i in range(iteno)]
     jj = len(ewp_idx)
-    dist = np.array(dist, float)
     ewno = min(jj, rppd)
     ritz_ev = np.zeros((iteno, rppd), complex)
     ritz_ew = np.zeros(rppd, complex)
     chosen = []
+    plain = abs(target_org - target) < 1.0e-10 or choose_type_org == "CL"
+    tor = float(np.real(target_org))
     for i in range(ewno):  # (:332-366)
         ind_sort = 0
         for k in range(1, jj):
             if dist[k] < dist[ind_sort]:
-                if abs(target_org - target) < 1.0e-10 or choose_type_org == "CL":
+                if plain:
                     ind_sort = k
                 elif choose_type_org == "RGTR":
-                    if ew[ewp_idx[k]].real > np.real(target_org) and abs(ew[ewp_idx[k]].imag) < 1.0e-12:
+                    e = ewl[ewp_idx[k]]
+                    if e.real > tor and abs(e.imag) < 1.0e-12:
                         ind_sort = k
                 elif choose_type_org == "RGTC":
-                    if ew[ewp_idx[k]].real > np.real(target_org):
+                    if ewl[ewp_idx[k]].real > tor:
                         ind_sort = k
         if dist[ind_sort] > 1.0e12:
-            ind_sort = int(np.argmin(dist[:jj]))
+            ind_sort = min(range(jj), key=dist.__getitem__)
         chosen.append(ewp_idx[ind_sort])
         ritz_ew[i] = ew[ewp_idx[ind_sort]]
         v = VRR[:iteno, ewp_idx[ind_sort]]
         ritz_ev[:, i] = v / np.linalg.norm(v)
         dist[ind_sort] = 1.0e14
     if ewno < rppd:  # fill with the remaining Ritz pairs closest to target (:371-411)
-        out_idx = [i for i in range(iteno) if i not in chosen]
-        cand = [i for i in range(iteno) if abs(sort_ritz[i]) < 1.0e10 and i in out_idx]
-        d2 = np.array([abs(ew[i] - target) for i in cand])
+        cand = [i for i in range(iteno) if abs(sort_ritz[i]) < 1.0e10 and i not in chosen]
+        d2 = np.array([abs(ewl[i] - target) for i in cand])
         order = np.argsort(d2, kind="stable")
         add = min(rppd - ewno, len(order))
         for ii in range(add):
@@ -223,7 +235,7 @@ class _State:
 
 def _projected_eig(M, L):
     """eig(M, L) for Hermitian M > 0 and Hermitian L: L y = mu M y, lambda = 1/mu."""
-    mu, Y = scipy.linalg.eigh(L, M)
+    mu, Y = scipy.linalg.eigh(L, M, check_finite=False)
     with np.errstate(divide="ignore"):
         lam = np.where(mu != 0, 1.0 / mu, np.inf).astype(complex)
     Y = Y / np.linalg.norm(Y, axis=0, keepdims=True)  # unit 2-norm columns, as eig's QZ path returns
