@@ -406,7 +406,7 @@ def run_config1(pkg, torch, args):
 
 def run_sweep(pkg, torch, args):
     """fncB matvecs/s at n = 24, 48, 60, 72 (SC woodpile, gamma = 0.2): batched over the 40-k
-    band path (autotuned plan) and plain (one k-point per call), algorithmic GB/s on the 512 B
+    band path and plain (one k-point per call), both plans autotuned, algorithmic GB/s on the 512 B
     per point model and the fraction of HBM peak; beside it the oracle's CPU fncB at n = 24,
     48, 60 (median of up to 20 calls after 3 warm-ups, capped at about --cpu-seconds per n)."""
     import ctypes
@@ -424,6 +424,7 @@ def run_sweep(pkg, torch, args):
         pb = pkg.Plan(dims, thetas, table, ids)
         p1 = pkg.Plan(dims, thetas[:1], table, ids)
         chunk, _ = pb.autotune()
+        p1.autotune()  # the plain path at its best too: launch shapes and material tile
         g = torch.Generator(device="cuda")
         g.manual_seed(n)
         xb = torch.randn(nk, 4 * N, dtype=torch.complex128, device="cuda", generator=g)

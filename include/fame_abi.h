@@ -300,8 +300,8 @@ int fame_fcc_set_generic(fame_fcc* plan, int generic);
 
 /* Launch shape of the fused NFGEP xy kernel (no reference counterpart; same results): split = 0
  * keeps the two planes of a work-group in LDS together, split = 1 transforms them one at a time
- * (one plane of LDS, so two work-groups share a CU; the default up to length 48); lengths above
- * 64 always use the first. */
+ * (one plane of LDS, so two work-groups share a CU; the default at lengths 40-48); lengths
+ * above 64 always use the first. */
 int fame_fcc_set_nfgep_variant(fame_fcc* plan, int split);
 
 /* v = p_mtx_T_prod_vec_FCC(v, FFT_parameter) on nvec vectors [nvec][n]
