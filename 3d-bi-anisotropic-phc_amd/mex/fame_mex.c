@@ -24,11 +24,19 @@
  *   [y, iters, relres] = fame_mex('minres', h, x, sigma, tol, maxit)     (fncB - sigma fncA) y = x
  *   fame_mex('vertex_material', h, eps, gamma, mu, lattice)              eps, gamma: N x 1, lattice 3x3
  *   g = fame_mex('material_gradient', h, ev)   ev: 4N x nvec (nb = 1) -> N x 3 x nvec
+ *   fame_mex('set_structure', h, kind, params, lattice)   device voxelisation (Material_Locate_Isofunction):
+ *        kind 0 woodpile {width, a}, 1 sphere {r, c1, c2, c3}, 2 tetrahedron {edge, c1, c2, c3};
+ *        lattice 3x3, lattice vectors as columns
+ *   ids = fame_mex('material_ids', h)               N x 1 uint8
  *   fame_mex('plan_destroy', h)
  *
  *   f = fame_mex('fcc_create', [n1 n2 n3], vec_k, a1, a2, a3, delta)     SVDSingleCurl_FCC tables
  *   y = fame_mex('fcc_T' | 'fcc_TH', f, x)                               x: n x nvec complex
- *   y = fame_mex('fcc_nfgep', f, x, xi_d, zeta_d, Phi)                  x: 32n, coefficients 24n
+ *   fame_mex('fcc_set_material', f, edge_mask, face_mask, tensors)     mtx_Eps_Mu_Zeta_Xi: masks n x 1
+ *        uint8, tensors 72 complex = [inside, air][eps, mu, zeta, xi] 3x3 row-major
+ *   [xi_d, zeta_d, Phi] = fame_mex('fcc_nfgep_data', f)                 mtx_data from GEP B (coupling 1)
+ *   y = fame_mex('fcc_nfgep', f, x, xi_d, zeta_d, Phi [, coupling])     x: 32n, coefficients 24n
+ *   ev = fame_mex('fcc_eigenvector', f, x, xi_d, zeta_d, Phi [, coupling])   Compute_eigenvector, 48n
  *   fame_mex('fcc_destroy', f)
  */
 #include <stdint.h>
@@ -147,17 +155,39 @@ static int fcc_command(const char* cmd, int nlhs, mxArray* plhs[], int nrhs, con
     check(fame_fcc_sync(f->plan));
     plhs[0] = mxCreateDoubleMatrix((mwSize)f->n, (mwSize)nvec, mxCOMPLEX);
     hip_check(hipMemcpy(mxGetComplexDoubles(plhs[0]), f->dout, cnt * 16, hipMemcpyDeviceToHost));
-  } else if (!strcmp(cmd, "fcc_nfgep")) {
-    const size_t n32 = 32 * (size_t)f->n, n24 = 24 * (size_t)f->n;
-    fstage(f, n32 * 16);
+  } else if (!strcmp(cmd, "fcc_nfgep") || !strcmp(cmd, "fcc_eigenvector")) {
+    if (nrhs != 6 && nrhs != 7) mexErrMsgIdAndTxt("fame:arg", "%s(f, x, xi_d, zeta_d, Phi [, coupling])", cmd);
+    const int eig = cmd[4] == 'e';
+    const size_t n32 = 32 * (size_t)f->n, n24 = 24 * (size_t)f->n, nout = eig ? 48 * (size_t)f->n : n32;
+    const int coupling = nrhs == 7 ? (int)mxGetScalar(prhs[6]) : 0;
+    fstage(f, nout * 16);
     void* c[3];
     for (int i = 0; i < 3; ++i) c[i] = dev_copy(prhs[3 + i], n24);
     if (!mxIsComplex(prhs[2]) || mxGetNumberOfElements(prhs[2]) != n32) mexErrMsgIdAndTxt("fame:arg", "x: 32n complex");
     hip_check(hipMemcpy(f->din, mxGetComplexDoubles(prhs[2]), n32 * 16, hipMemcpyHostToDevice));
-    check(fame_fcc_nfgep_A(f->plan, f->din, f->dout, c[0], c[1], c[2]));
+    check(eig ? fame_fcc_eigenvector_ex(f->plan, f->din, f->dout, c[0], c[1], c[2], coupling)
+              : fame_fcc_nfgep_A_ex(f->plan, f->din, f->dout, c[0], c[1], c[2], coupling));
     check(fame_fcc_sync(f->plan));
-    plhs[0] = mxCreateDoubleMatrix((mwSize)n32, 1, mxCOMPLEX);
-    hip_check(hipMemcpy(mxGetComplexDoubles(plhs[0]), f->dout, n32 * 16, hipMemcpyDeviceToHost));
+    plhs[0] = mxCreateDoubleMatrix((mwSize)nout, 1, mxCOMPLEX);
+    hip_check(hipMemcpy(mxGetComplexDoubles(plhs[0]), f->dout, nout * 16, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; ++i) hipFree(c[i]);
+  } else if (!strcmp(cmd, "fcc_set_material")) {
+    if (nrhs != 5 || !mxIsComplex(prhs[4]) || mxGetNumberOfElements(prhs[4]) != 72 ||
+        mxGetNumberOfElements(prhs[2]) != (size_t)f->n || mxGetNumberOfElements(prhs[3]) != (size_t)f->n)
+      mexErrMsgIdAndTxt("fame:arg", "fcc_set_material(f, edge_mask (n uint8), face_mask (n uint8), tensors (72 complex))");
+    check(fame_fcc_set_material(f->plan, (const unsigned char*)mxGetData(prhs[2]),
+                                (const unsigned char*)mxGetData(prhs[3]),
+                                (const double*)mxGetComplexDoubles(prhs[4])));
+  } else if (!strcmp(cmd, "fcc_nfgep_data")) {
+    const size_t n24 = 24 * (size_t)f->n;
+    void* c[3];
+    for (int i = 0; i < 3; ++i) hip_check(hipMalloc(&c[i], n24 * 16));
+    check(fame_fcc_nfgep_data(f->plan, c[0], c[1], c[2]));
+    check(fame_fcc_sync(f->plan));
+    for (int i = 0; i < 3 && i < (nlhs > 0 ? nlhs : 1); ++i) {
+      plhs[i] = mxCreateDoubleMatrix((mwSize)n24, 1, mxCOMPLEX);
+      hip_check(hipMemcpy(mxGetComplexDoubles(plhs[i]), c[i], n24 * 16, hipMemcpyDeviceToHost));
+    }
     for (int i = 0; i < 3; ++i) hipFree(c[i]);
   } else {
     mexErrMsgIdAndTxt("fame:arg", "unknown command %s", cmd);
@@ -294,6 +324,22 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) A[3 * i + j] = L[i + 3 * j];
     check(fame_plan_set_vertex_material(s->plan, mxGetDoubles(prhs[2]), mxGetDoubles(prhs[3]), mxGetScalar(prhs[4]), A));
+  } else if (!strcmp(cmd, "set_structure")) {
+    if (nrhs != 5 || mxGetNumberOfElements(prhs[4]) != 9)
+      mexErrMsgIdAndTxt("fame:arg", "set_structure(h, kind, params, lattice 3x3)");
+    const double* L = mxGetDoubles(prhs[4]);
+    double A[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) A[3 * i + j] = L[i + 3 * j];
+    check(fame_plan_set_structure(s->plan, (int)mxGetScalar(prhs[2]), mxGetDoubles(prhs[3]), A));
+  } else if (!strcmp(cmd, "material_ids")) {
+    void* d = NULL;
+    hip_check(hipMalloc(&d, (size_t)N));
+    check(fame_plan_material_ids(s->plan, d));
+    check(fame_sync(s->plan));
+    plhs[0] = mxCreateNumericMatrix((mwSize)N, 1, mxUINT8_CLASS, mxREAL);
+    hip_check(hipMemcpy(mxGetData(plhs[0]), d, (size_t)N, hipMemcpyDeviceToHost));
+    hipFree(d);
   } else if (!strcmp(cmd, "material_gradient")) {
     if (s->nb != 1) mexErrMsgIdAndTxt("fame:arg", "material_gradient: plans with one k-point");
     const int nvec = (int)mxGetN(prhs[2]);

@@ -18,7 +18,7 @@ typedef size_t mwSize;
 typedef struct mxArray_tag mxArray;
 typedef struct { double real, imag; } mxComplexDouble;
 typedef enum { mxREAL, mxCOMPLEX } mxComplexity;
-typedef enum { mxDOUBLE_CLASS, mxUINT64_CLASS } mxClassID;
+typedef enum { mxDOUBLE_CLASS, mxUINT8_CLASS, mxUINT64_CLASS } mxClassID;
 void mexErrMsgIdAndTxt(const char*, const char*, ...);
 int mxIsUint64(const mxArray*); int mxIsComplex(const mxArray*);
 size_t mxGetNumberOfElements(const mxArray*); void* mxGetData(const mxArray*);
