@@ -431,6 +431,41 @@ def test_launch_variants_identical(pkg, torch_mod, n):
     plan.close()
 
 
+@pytest.mark.parametrize("n", [24, 60])
+def test_material_tiles_identical(pkg, torch_mod, n):
+    """Every material-stencil tile and slab length gives the same fncB bit for bit (ragged edge
+    tiles and one-slice slabs included), with table media and with per-vertex media; a tile
+    beyond one 256-thread work-group is refused; the fused CG on a non-default tile solves to
+    its tolerance."""
+    say(f"material tiles n={n}")
+    S = pkg.structures
+    dims = (n, n, n)
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in S.sc_band_path()[[3, 17]]])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    x = torch_mod.as_tensor(crand(2, 4 * plan.N))
+    ref = plan.fncB(x).cpu().numpy()
+    tiles = [(15, 12, 5), (14, 14, 3), (12, 15, 60), (20, 9, 7), (30, 5, 2), (7, 11, 1), (6, 6, 4)]
+    for tx, ty, tz in tiles:
+        tx, ty, tz = min(tx, n), min(ty, n), min(tz, n)
+        plan.set_material_tile(tx, ty, tz)
+        assert np.array_equal(plan.fncB(x).cpu().numpy(), ref), (tx, ty, tz)
+    with pytest.raises(pkg.FameError, match="256"):
+        plan.set_material_tile(20, 20, 2)
+    plan.set_material_tile(20, 9, 3)
+    b = plan.fncB(x)
+    y, it, rr = plan.fncinvB(b, tol=1e-12)
+    assert float((y.cpu() - x).norm() / x.norm()) < 1e-7 and np.all(rr <= 1e-12)
+    rng = np.random.default_rng(7)
+    plan.set_vertex_material(rng.uniform(1, 13, plan.N), np.full(plan.N, 0.2))
+    plan.set_material_tile(0, 0)
+    refv = plan.fncB(x).cpu().numpy()
+    for tx, ty, tz in [(20, 9, 3), (14, 14, 5)]:
+        plan.set_material_tile(tx, ty, tz)
+        assert np.array_equal(plan.fncB(x).cpu().numpy(), refv), (tx, ty, tz)
+    plan.close()
+
+
 def test_batched_lanczos_equals_plain(pkg, torch_mod):
     """Lock-step batched Lanczos (k-points finishing at different steps, solved on a
     packed view of the unfinished ones) gives each k-point's plain result."""
