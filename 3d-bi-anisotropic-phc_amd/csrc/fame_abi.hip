@@ -91,6 +91,8 @@ struct fame_plan {
   double2* d_mr = nullptr;    // MINRES work vectors [6][nb][4N], allocated by the first fame_minres
   void* d_mrst = nullptr;     // MINRES per-item state [2][nb] (the iteration alternates halves)
   double2* d_mrp = nullptr;   // MINRES partial sums [3][nb][2048] and alpha [nb]
+  double2* d_mrm = nullptr;   // MINRES: the material pass's partial sums of u^H A u [nb][mrm_stride]
+  size_t mrm_cap = 0;
   double2* d_mgp = nullptr;   // fame_mgs partial sums [2][1024]
   double2* d_mgs = nullptr;   // fame_mgs coefficients when the caller passes none
   int mgs_cap = 0;
@@ -550,7 +552,10 @@ int twist(fame_plan* P, const double2* in, double2* out, int ncomp, int conj_twi
 
 // One fncB over batch items [b0, b0 + nbc): z-pass + Pi mix, xy-pass, material, xy-pass, z-pass + Pi^H.
 // With cf (fused CG), x is the CG residual r and the transformed vector is p <- r + beta p.
-int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, const CgFuse* cf = nullptr) {
+// mpart (optional, without cf): the material pass leaves its per-work-group partial sums of
+// u^H (Mat u) there, item stride mpstride (for an isometric transform chain this is x^H fncB x)
+int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, const CgFuse* cf = nullptr,
+                double2* mpart = nullptr, int mpstride = 0) {
   const long long N = P->N;
   struct Window {  // restores the full-batch window on every exit path
     fame_plan* P;
@@ -584,7 +589,8 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   // passes (1.2 % faster fncB at n = 60, 40 k, same results bit for bit)
   RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w1, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
-  RET_IF(material(P, P->d_w1, 6 * N, P->d_w2, 6 * N, pcf ? wcf.part : nullptr, pcf ? wcf.nparts : 0, cgst));
+  double2* mp = pcf ? wcf.part : (mpart ? mpart + (size_t)b0 * mpstride : nullptr);
+  RET_IF(material(P, P->d_w1, 6 * N, P->d_w2, 6 * N, mp, pcf ? wcf.nparts : mpstride, cgst));
   if (pcf) {
     hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, material_parts(P, nbc), wcf.nparts,
                        wcf.st);
@@ -608,10 +614,11 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
 
 // fncB on all nb items, in chunks so the intermediate fields of a chunk can stay
 // in the device's last-level cache between the five kernels.
-int do_fncB(fame_plan* P, const double2* x, double2* y) {
+int do_fncB(fame_plan* P, const double2* x, double2* y, double2* mpart = nullptr, int mpstride = 0) {
   const int nb = P->vnb;
   const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
-  for (int b0 = 0; b0 < nb; b0 += c) RET_IF(fncB_window(P, b0, std::min(c, nb - b0), x, y));
+  for (int b0 = 0; b0 < nb; b0 += c)
+    RET_IF(fncB_window(P, b0, std::min(c, nb - b0), x, y, nullptr, mpart, mpstride));
   if (P->prof) P->prof_calls += 1;
   return FAME_OK;
 }
@@ -1033,7 +1040,7 @@ int fame_plan_destroy(fame_plan* P) {
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
                   P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs, P->d_bdp, P->d_jdy, P->d_mrp,
-                  P->d_mgp};
+                  P->d_mgp, P->d_mrm};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
