@@ -138,6 +138,35 @@ def test_minres_iterates_match_oracle(pkg, fo, torch_mod, sigma):
     plan.close()
 
 
+@pytest.mark.parametrize("sigma", [0.0, 0.8])
+def test_minres_stencil_partials_ragged_chunks(pkg, fo, torch_mod, sigma):
+    """On a specialised grid MINRES takes u^H A u from the material pass's partial sums and
+    u^H fncA u from its Lanczos update; with three k-points in chunks of two the last chunk's
+    stencil leaves a different number of partials.  Iterates after 1, 4 and 15 steps equal the
+    oracle's MINRES (1e-10), and the generic-stencil path (one more inner-product kernel) to
+    1e-12."""
+    say(f"minres stencil partials sigma={sigma}")
+    dims = (8, 8, 8)
+    thetas = np.vstack([THETAS, [[0.9, 0.6, -1.9]]])
+    plan, probs = woodpile_plan(pkg, fo, dims, thetas)
+    plan.set_chunk(2)
+    x = crand(3, 4 * plan.N)
+    for steps in (1, 4, 15):
+        y, iters, relres = plan.minres(torch_mod.as_tensor(x), sigma=sigma, tol=1e-30, maxit=steps)
+        y = y.cpu().numpy()
+        plan.set_generic(True)
+        yg, itg, _ = plan.minres(torch_mod.as_tensor(x), sigma=sigma, tol=1e-30, maxit=steps)
+        plan.set_generic(False)
+        for b in range(3):
+            P = probs[b]
+            yo, flag, ro, ito = fo.minres(lambda v: P.fncB(v) - sigma * P.fncA(v), x[b], 1e-30, steps)
+            assert iters[b] == steps == ito == itg[b]
+            assert rel(y[b], yo) < 1e-10, (steps, b)
+            assert rel(y[b], yg[b].cpu().numpy()) < 1e-12, (steps, b)
+            assert abs(relres[b] - ro) <= 1e-8 * ro
+    plan.close()
+
+
 def test_minres_converges(pkg, fo, torch_mod):
     say("minres converged solve, batched")
     dims = (8, 8, 8)
