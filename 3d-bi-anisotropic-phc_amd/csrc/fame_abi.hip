@@ -2,6 +2,7 @@
 // launch sequences, CG (fncinvB) and Lanczos, behind the C-ABI of
 // include/fame_abi.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -144,6 +145,10 @@ struct fame_plan {
   hipEvent_t ev[6] = {};
   double stage_ms[5] = {0, 0, 0, 0, 0};
   long long prof_calls = 0;
+  hipEvent_t evk[2] = {};      // the material stencil's own launch events (profiling)
+  bool evk_used = false;       // the last fncB window launched the stencil with them
+  double mat_kernel_ms = 0;    // accumulated stencil kernel time (profiling)
+  long long mat_launches = 0;
 };
 
 namespace {
@@ -519,18 +524,32 @@ int material(fame_plan* P, const double2* in, long long ibs, double2* out, long 
     material_shape(P, P->cnb, TX, TY, TZ, ntile, nzc);
     if (dpart && ntile * nzc > pstride) return set_err(FAME_ERR_ARG, "material: partial-sum stride too small");
     const dim3 grid(ntile, nzc, P->cnb);
+    // profiling: the launch's own start / stop events time the kernel alone (as a kernel trace
+    // does), without the queue gaps that events recorded between launches include
+    const double* nulld = nullptr;
+#define MZ_GO(PV_, EPS_, GAM_, COEF_)                                                                          \
+  do {                                                                                                         \
+    const std::uint32_t lds_ = (std::uint32_t)material_z_lds(TX, TY, PV_);                                   \
+    if (P->prof)                                                                                               \
+      hipExtLaunchKernelGGL(k_material_z<PV_>, grid, dim3(kMzThreads), lds_, P->stream, P->evk[0], P->evk[1], \
+                            0u, P->geo, in, ibs, out, obs, (const unsigned char*)P->d_mat_id,                  \
+                            (const double2*)P->d_mat, P->n_mat, EPS_, GAM_, COEF_, (const double2*)wom(P), TX, \
+                            TY, TZ, dpart, pstride, cgst);                                                     \
+    else                                                                                                       \
+      hipLaunchKernelGGL(k_material_z<PV_>, grid, dim3(kMzThreads), lds_, P->stream, P->geo, in, ibs, out, obs, \
+                         (const unsigned char*)P->d_mat_id, (const double2*)P->d_mat, P->n_mat, EPS_, GAM_,    \
+                         COEF_, (const double2*)wom(P), TX, TY, TZ, dpart, pstride, cgst);                     \
+  } while (0)
     if (P->pv) {
       PvCoef c;
       for (int i = 0; i < 9; ++i) c.G[i] = P->pv_G[i];
       c.mu = P->pv_mu;
-      hipLaunchKernelGGL(k_material_z<true>, grid, dim3(kMzThreads), material_z_lds(TX, TY, true), P->stream, P->geo,
-                         in, ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, P->d_eps, P->d_gam, c, wom(P), TX, TY,
-                         TZ, dpart, pstride, cgst);
+      MZ_GO(true, (const double*)P->d_eps, (const double*)P->d_gam, c);
     } else {
-      hipLaunchKernelGGL(k_material_z<false>, grid, dim3(kMzThreads), material_z_lds(TX, TY), P->stream, P->geo, in,
-                         ibs, out, obs, P->d_mat_id, P->d_mat, P->n_mat, nullptr, nullptr, PvCoef{}, wom(P), TX, TY,
-                         TZ, dpart, pstride, cgst);
+      MZ_GO(false, nulld, nulld, PvCoef{});
     }
+#undef MZ_GO
+    P->evk_used = P->prof;
     CHECK_LAUNCH();
     return FAME_OK;
   }
@@ -603,6 +622,13 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   if (pr) {
     HIP_TRY(hipEventRecord(P->ev[5], P->stream));
     HIP_TRY(hipEventSynchronize(P->ev[5]));
+    if (P->evk_used) {
+      float km = 0;
+      HIP_TRY(hipEventElapsedTime(&km, P->evk[0], P->evk[1]));
+      P->mat_kernel_ms += km;
+      P->mat_launches += 1;
+      P->evk_used = false;
+    }
     for (int s = 0; s < 5; ++s) {
       float ms = 0;
       HIP_TRY(hipEventElapsedTime(&ms, P->ev[s], P->ev[s + 1]));
@@ -1045,6 +1071,8 @@ int fame_plan_destroy(fame_plan* P) {
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : P->evk)
+    if (e) (void)hipEventDestroy(e);
   if (P->ev_poll) (void)hipEventDestroy(P->ev_poll);
   if (P->h_cg) (void)hipHostFree(P->h_cg);
   if (P->own_stream) {
@@ -1254,11 +1282,23 @@ int fame_plan_set_cg_fused(fame_plan* P, int fused) {
 
 int fame_plan_profile(fame_plan* P, int enable) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
-  if (enable && !P->ev[0])
+  if (enable && !P->ev[0]) {
     for (auto& e : P->ev) HIP_TRY(hipEventCreate(&e));
+    for (auto& e : P->evk) HIP_TRY(hipEventCreate(&e));
+  }
   P->prof = enable != 0;
   for (double& v : P->stage_ms) v = 0;
   P->prof_calls = 0;
+  P->mat_kernel_ms = 0;
+  P->mat_launches = 0;
+  P->evk_used = false;
+  return FAME_OK;
+}
+
+int fame_plan_material_kernel_time(fame_plan* P, double* ms, long long* launches) {
+  if (!P || !ms) return set_err(FAME_ERR_ARG, "NULL argument");
+  *ms = P->mat_kernel_ms;
+  if (launches) *launches = P->mat_launches;
   return FAME_OK;
 }
 

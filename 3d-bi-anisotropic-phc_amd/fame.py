@@ -100,6 +100,7 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_set_variant": ([vp, i, i], i),
         "fame_plan_get_variant": ([vp, pi_, pi_], i),
         "fame_plan_stage_times": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
+        "fame_plan_material_kernel_time": ([vp, pd, ctypes.POINTER(ctypes.c_longlong)], i),
         "fame_plan_set_vertex_material": ([vp, pd, pd, d, pd], i),
         "fame_material_gradient": ([vp, i, vp, vp], i),
         "fame_stream_probe": ([i, vp, vp, ctypes.c_longlong, i, vp, pd], i),
@@ -141,7 +142,7 @@ def exported_symbols():
         "fame_sync", "fame_plan_stream", "fame_plan_set_stream", "fame_MtxVecMul_FFT", "fame_Pr", "fame_Qr",
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs", "fame_block_dot_noalloc", "fame_jd_ritz_residual", "fame_jd_expand",
-        "fame_plan_profile", "fame_plan_stage_times", "fame_plan_set_cg_fused", "fame_plan_set_cg_graph",
+        "fame_plan_profile", "fame_plan_stage_times", "fame_plan_material_kernel_time", "fame_plan_set_cg_fused", "fame_plan_set_cg_graph",
         "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
@@ -322,6 +323,14 @@ class Plan:
         calls = ctypes.c_longlong()
         _check(self.lib.fame_plan_stage_times(self.h, ms, ctypes.byref(calls)))
         return dict(zip(self.STAGES, ms[:])), calls.value
+
+    def material_kernel_time(self):
+        """(ms, launches): the material stencil's own kernel time while profiling (its launch
+        events), without the gaps the stage events include."""
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        _check(self.lib.fame_plan_material_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     # -- buffers ------------------------------------------------------------------
     def empty(self, ncomp):

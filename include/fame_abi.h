@@ -141,6 +141,11 @@ int fame_plan_set_cg_graph(fame_plan* plan, int on);
  * milliseconds per stage (ms[5]) and the number of profiled calls. */
 int fame_plan_profile(fame_plan* plan, int enable);
 int fame_plan_stage_times(fame_plan* plan, double* ms, long long* calls);
+/* Measurement aid (no reference counterpart): while profiling, the material stencil of the
+ * specialised path is launched with its own start / stop events (hipExtLaunchKernelGGL), which
+ * time the kernel alone as a kernel trace does; returns the accumulated milliseconds and the
+ * number of timed launches. */
+int fame_plan_material_kernel_time(fame_plan* plan, double* ms, long long* launches);
 
 /* Psi / Phi ("notranspose", istrans = 0) and Psis / Phis ("transpose", istrans = 1)
  * on a 3-component field [nb][3N]:  y_c = S .* (ifftn(x_c) * sqrt(N)),
