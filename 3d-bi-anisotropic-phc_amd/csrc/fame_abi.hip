@@ -69,6 +69,7 @@ struct fame_plan {
   int n1 = 0, n2 = 0, n3 = 0, nb = 0;
   long long N = 0;
   int device = 0;                    // the HIP device current when the plan was created
+  int ncu = 256;                     // compute units of that device
   hipStream_t stream = nullptr;      // stream in use
   hipStream_t own_stream = nullptr;  // stream created by the plan
   Geo geo{};
@@ -122,6 +123,13 @@ struct fame_plan {
   CgState* h_cg = nullptr;    // pinned host mirror of d_cg for the lagged convergence poll
   int h_cg_cap = 0;
   hipEvent_t ev_poll = nullptr;
+  // second stream for chunk pairs: odd chunks of fncB and of a lock-step CG iteration run on
+  // stream2 with their own half of the field scratch, so one chunk's kernels fill the other's
+  // launch tails; fork / join by events on the caller's stream
+  int nstreams = 1;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int w_off = 0;              // items into d_w1 / d_w2 where the current window's fields live
   int tab_len = 0;            // n1+n2+n3
   bool fast = true;           // use the specialised FFT kernels where a size has them
   // batch window used by the launch helpers: items [cb0, cb0 + cnb) of the current view
@@ -466,9 +474,20 @@ int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, d
   return FAME_OK;
 }
 
+// Slab length of the 2.5-D material kernel: as few slabs per tile column as fill one round of
+// work-groups (tiles * nbc * nzc <= three 256-thread work-groups per compute unit, what fits at
+// the kernel's registers and LDS), since every slab re-reads one slice below and one above it;
+// at least one slab, slabs of at least three slices.
+int slab_length(int n1, int n2, int n3, int TX, int TY, long long nbc, int ncu) {
+  const long long cols = (long long)((n1 + TX - 1) / TX) * ((n2 + TY - 1) / TY) * nbc;
+  long long nzc = std::max<long long>(1, 3LL * ncu / cols);
+  nzc = std::min<long long>(nzc, std::max(1, n3 / 3));
+  return (int)((n3 + nzc - 1) / nzc);
+}
+
 // Tile of the 2.5-D material kernel: maximise (useful columns) x (tile / tile-with-halo) under
-// (TX+2)(TY+2) <= kMzThreads; z-slabs so that a launch has enough work-groups.
-void material_tile(int n1, int n2, int n3, int nbc, int& TX, int& TY, int& TZ) {
+// (TX+2)(TY+2) <= kMzThreads; the slab length as above.
+void material_tile(int n1, int n2, int n3, int nbc, int ncu, int& TX, int& TY, int& TZ) {
   double best = -1;
   TX = TY = 1;
   for (int tx = 1; tx <= std::min(n1, kMzThreads); ++tx)
@@ -482,12 +501,7 @@ void material_tile(int n1, int n2, int n3, int nbc, int& TX, int& TY, int& TZ) {
         TY = ty;
       }
     }
-  // slabs: about 2048 work-groups in all, slabs of at least 3 slices (each slab re-reads
-  // one slice below and one above)
-  const long long tiles = (long long)((n1 + TX - 1) / TX) * ((n2 + TY - 1) / TY) * nbc;
-  long long nzc = (2048 + tiles - 1) / tiles;
-  nzc = std::max<long long>(1, std::min<long long>(nzc, std::max(1, n3 / 3)));
-  TZ = (int)((n3 + nzc - 1) / nzc);
+  TZ = slab_length(n1, n2, n3, TX, TY, nbc, ncu);
 }
 
 int pv_material(fame_plan* P, const double2* in, long long ibs, double2* out, long long obs, double2* dpart,
@@ -495,7 +509,7 @@ int pv_material(fame_plan* P, const double2* in, long long ibs, double2* out, lo
 
 // tile, slab and grid of the 2.5-D material kernel for a launch of cnb items
 void material_shape(const fame_plan* P, int cnb, int& TX, int& TY, int& TZ, int& ntile, int& nzc) {
-  material_tile(P->n1, P->n2, P->n3, cnb, TX, TY, TZ);
+  material_tile(P->n1, P->n2, P->n3, cnb, P->ncu, TX, TY, TZ);
   if (P->mt[0] > 0) {  // tile chosen by the autotuner or the caller
     TX = P->mt[0];
     TY = P->mt[1];
@@ -598,7 +612,9 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
   // CG: the material pass sums p^H q (in real space), alpha is formed right after it, and the
   // MIX_OUT pass updates r in place (y is r then)
-  RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, P->d_w1, 6 * N, 2, 0, pcf));
+  double2* w1 = P->d_w1 + (size_t)P->w_off * 6 * N;
+  double2* w2 = P->d_w2 + (size_t)P->w_off * 6 * N;
+  RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, w1, 6 * N, 2, 0, pcf));
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
   // converged items of a fused CG are skipped by every pass (their state is frozen)
   const CgState* cgst = pcf ? wcf.st : nullptr;
@@ -606,19 +622,19 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   // and stores it after its last): a chunk's fields then occupy two buffers that are each read
   // and rewritten on the same lines, which keeps more of them in the Infinity Cache between
   // passes (1.2 % faster fncB at n = 60, 40 k, same results bit for bit)
-  RET_IF(xypass(P, +1, 0, P->d_w1, 6 * N, P->d_w1, 6 * N, 6, cgst));
+  RET_IF(xypass(P, +1, 0, w1, 6 * N, w1, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
   double2* mp = pcf ? wcf.part : (mpart ? mpart + (size_t)b0 * mpstride : nullptr);
-  RET_IF(material(P, P->d_w1, 6 * N, P->d_w2, 6 * N, mp, pcf ? wcf.nparts : mpstride, cgst));
+  RET_IF(material(P, w1, 6 * N, w2, 6 * N, mp, pcf ? wcf.nparts : mpstride, cgst));
   if (pcf) {
     hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, material_parts(P, nbc), wcf.nparts,
                        wcf.st);
     CHECK_LAUNCH();
   }
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
-  RET_IF(xypass(P, -1, 0, P->d_w2, 6 * N, P->d_w2, 6 * N, 6, cgst));
+  RET_IF(xypass(P, -1, 0, w2, 6 * N, w2, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
-  RET_IF(zpass(P, -1, ZM_MIX_OUT, P->d_w2, 6 * N, y, 4 * N, 2, 0, pcf));
+  RET_IF(zpass(P, -1, ZM_MIX_OUT, w2, 6 * N, y, 4 * N, 2, 0, pcf));
   if (pr) {
     HIP_TRY(hipEventRecord(P->ev[5], P->stream));
     HIP_TRY(hipEventSynchronize(P->ev[5]));
@@ -640,11 +656,50 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
 
 // fncB on all nb items, in chunks so the intermediate fields of a chunk can stay
 // in the device's last-level cache between the five kernels.
+// Two-stream chunk schedule: chunk k runs on the plan's stream (k even) or on stream2 (k odd)
+// with fields in scratch items [(k % 2) c, ...); both streams start after the work queued so far
+// (fork) and the caller's stream waits for stream2 at the end (join).  Chunks touch disjoint items,
+// so the only ordering they need is each chunk's own, which its stream keeps.  Off while
+// profiling (the per-kernel events assume one stream) and for a single chunk.
+struct PairSchedule {
+  fame_plan* P;
+  hipStream_t main;
+  bool on = false;
+  explicit PairSchedule(fame_plan* p) : P(p), main(p->stream) {}
+  int begin(int nchunks) {
+    on = P->nstreams == 2 && !P->prof && nchunks > 1;
+    if (!on) return FAME_OK;
+    HIP_TRY(hipEventRecord(P->ev_fork, main));
+    HIP_TRY(hipStreamWaitEvent(P->stream2, P->ev_fork, 0));
+    return FAME_OK;
+  }
+  void select(int k, int c) {  // before chunk k (chunk size c) is queued
+    if (!on) return;
+    P->stream = (k % 2) ? P->stream2 : main;
+    P->w_off = (k % 2) * c;
+  }
+  int end() {  // restores the plan's stream; the caller's stream then waits for stream2
+    P->stream = main;
+    P->w_off = 0;
+    if (!on) return FAME_OK;
+    on = false;
+    HIP_TRY(hipEventRecord(P->ev_join, P->stream2));
+    HIP_TRY(hipStreamWaitEvent(main, P->ev_join, 0));
+    return FAME_OK;
+  }
+  ~PairSchedule() { (void)end(); }
+};
+
 int do_fncB(fame_plan* P, const double2* x, double2* y, double2* mpart = nullptr, int mpstride = 0) {
   const int nb = P->vnb;
   const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
-  for (int b0 = 0; b0 < nb; b0 += c)
+  PairSchedule ps(P);
+  RET_IF(ps.begin((nb + c - 1) / c));
+  for (int b0 = 0, k = 0; b0 < nb; b0 += c, ++k) {
+    ps.select(k, c);
     RET_IF(fncB_window(P, b0, std::min(c, nb - b0), x, y, nullptr, mpart, mpstride));
+  }
+  RET_IF(ps.end());
   if (P->prof) P->prof_calls += 1;
   return FAME_OK;
 }
@@ -856,9 +911,19 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
     bool done = false, pending = false;
     for (int it = 0; it < maxit && !done;) {
       const int batch = std::min(4, maxit - it);
-      for (int k = 0; k < batch; ++k) {
-        for (int b0 = 0; b0 < nb; b0 += c) RET_IF(step(b0, std::min(c, nb - b0)));
-        if (P->prof) P->prof_calls += 1;
+      {
+        // a chunk's iterations depend only on that chunk: with two streams each keeps its chunks
+        // for the whole batch, joined before the state copy below
+        PairSchedule ps(P);
+        RET_IF(ps.begin((nb + c - 1) / c));
+        for (int k = 0; k < batch; ++k) {
+          for (int b0 = 0, ck = 0; b0 < nb; b0 += c, ++ck) {
+            ps.select(ck, c);
+            RET_IF(step(b0, std::min(c, nb - b0)));
+          }
+          if (P->prof) P->prof_calls += 1;
+        }
+        RET_IF(ps.end());
       }
       it += batch;
       if (pending) {  // the state after the previous batch
@@ -975,6 +1040,8 @@ int fame_plan_create(fame_plan** plan, int n1, int n2, int n3, int nb, const dou
   }
   auto fail = [&](int rc) { fame_plan_destroy(P); return rc; };
   if (hipGetDevice(&P->device) != hipSuccess) return fail(set_err(FAME_ERR_HIP, "hipGetDevice failed"));
+  if (hipDeviceGetAttribute(&P->ncu, hipDeviceAttributeMultiprocessorCount, P->device) != hipSuccess || P->ncu < 1)
+    return fail(set_err(FAME_ERR_HIP, "compute unit count"));
 #define PLAN_HIP(call)                                                                              \
   do {                                                                                              \
     hipError_t e_ = (call);                                                                         \
@@ -1074,6 +1141,12 @@ int fame_plan_destroy(fame_plan* P) {
   for (hipEvent_t e : P->evk)
     if (e) (void)hipEventDestroy(e);
   if (P->ev_poll) (void)hipEventDestroy(P->ev_poll);
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  if (P->stream2) {
+    (void)hipStreamSynchronize(P->stream2);
+    (void)hipStreamDestroy(P->stream2);
+  }
   if (P->h_cg) (void)hipHostFree(P->h_cg);
   if (P->own_stream) {
     (void)hipStreamSynchronize(P->own_stream);
@@ -1131,9 +1204,10 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
     double2 *x = nullptr, *y = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool prof, ok = false;
-    int xyv, zv, chunk, mt[3];
+    int xyv, zv, chunk, mt[3], ns;
     explicit Guard(fame_plan* p)
-        : P(p), prof(p->prof), xyv(p->xyv), zv(p->zv), chunk(p->chunk), mt{p->mt[0], p->mt[1], p->mt[2]} {}
+        : P(p), prof(p->prof), xyv(p->xyv), zv(p->zv), chunk(p->chunk), mt{p->mt[0], p->mt[1], p->mt[2]},
+          ns(p->nstreams) {}
     ~Guard() {
       if (e0) (void)hipEventDestroy(e0);
       if (e1) (void)hipEventDestroy(e1);
@@ -1145,6 +1219,7 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
         P->xyv = xyv;
         P->zv = zv;
         P->chunk = chunk;
+        P->nstreams = ns;
         for (int i = 0; i < 3; ++i) P->mt[i] = mt[i];
       }
     }
@@ -1160,16 +1235,20 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   HIP_TRY(hipEventCreate(&G.e1));
   hipEvent_t e0 = G.e0, e1 = G.e1;
   P->prof = false;
+  // the best of three timings of five calls each (a single timing varies by a few per cent)
   auto time_it = [&](double& ms_out) -> int {
     RET_IF(do_fncB(P, x, y));  // warm-up
-    HIP_TRY(hipEventRecord(e0, P->stream));
-    const int reps = 5;
-    for (int r = 0; r < reps; ++r) RET_IF(do_fncB(P, x, y));
-    HIP_TRY(hipEventRecord(e1, P->stream));
-    HIP_TRY(hipEventSynchronize(e1));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    ms_out = ms / reps;
+    ms_out = 1e30;
+    for (int trial = 0; trial < 3; ++trial) {
+      HIP_TRY(hipEventRecord(e0, P->stream));
+      const int reps = 5;
+      for (int r = 0; r < reps; ++r) RET_IF(do_fncB(P, x, y));
+      HIP_TRY(hipEventRecord(e1, P->stream));
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      ms_out = std::min(ms_out, (double)ms / reps);
+    }
     return FAME_OK;
   };
   double best = 1e30;
@@ -1189,14 +1268,22 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   std::vector<int> cands;
   for (int c : {1, 2, 3, 4, 6, 8, 12, 16})
     if (c < P->nb) cands.push_back(c);
-  for (int c : cands) {
-    P->chunk = c;
-    double ms;
-    RET_IF(time_it(ms));
-    if (ms < best) { best = ms; bestc = c; }
-  }
+  // each chunk size on one stream, and on two (chunk pairs overlap) where there are two chunks
+  P->nstreams = 1;
+  int bestns = 1;
+  if (!cands.empty()) RET_IF(fame_plan_set_streams(P, 1));
+  for (int c : cands)
+    for (int ns = 1; ns <= 2; ++ns) {
+      if (ns == 2) RET_IF(fame_plan_set_streams(P, 2));
+      P->nstreams = ns;
+      P->chunk = c;
+      double ms;
+      RET_IF(time_it(ms));
+      if (ms < best) { best = ms; bestc = c; bestns = ns; }
+    }
   P->chunk = bestc;
-  // material stencil tile and z-slab: the heuristic's best few shapes, each with three slab lengths
+  P->nstreams = bestns;
+  // material stencil tile and z-slab: the heuristic's best few shapes, four slab lengths each
   if (P->fast) {
     struct Cand { double score; int tx, ty; };
     std::vector<Cand> tiles;
@@ -1217,11 +1304,12 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
           for (const Cand& c : tiles) dup |= (c.tx == tx && c.ty == ty);
           if (!dup) tiles.push_back({0.0, tx, ty});
         }
-    int hx, hy, hz;
-    material_tile(P->n1, P->n2, P->n3, (bestc > 0 ? bestc : P->nb), hx, hy, hz);
+    const long long nbc = bestc > 0 ? bestc : P->nb;
     int btx = 0, bty = 0, btz = 0;
-    for (const Cand& c : tiles)
-      for (int tz : {hz, std::max(1, hz / 2), std::min(P->n3, 2 * hz)}) {
+    // slab lengths: about one, two or three rounds of work-groups, or half a round
+    for (const Cand& c : tiles) {
+      const int hz = slab_length(P->n1, P->n2, P->n3, c.tx, c.ty, nbc, P->ncu);
+      for (int tz : {hz, std::max(1, hz / 2), std::max(1, hz / 3), std::min(P->n3, 2 * hz)}) {
         P->mt[0] = c.tx;
         P->mt[1] = c.ty;
         P->mt[2] = tz;
@@ -1229,6 +1317,7 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
         RET_IF(time_it(ms));
         if (ms < best) { best = ms; btx = c.tx; bty = c.ty; btz = tz; }
       }
+    }
     P->mt[0] = btx;
     P->mt[1] = bty;
     P->mt[2] = btz;
@@ -1247,6 +1336,24 @@ int fame_plan_set_generic(fame_plan* P, int generic) {
 int fame_plan_set_cg_graph(fame_plan* P, int on) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
   P->cg_graph = (on != 0);
+  return FAME_OK;
+}
+
+int fame_plan_set_streams(fame_plan* P, int nstreams) {
+  if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
+  if (nstreams != 1 && nstreams != 2) return set_err(FAME_ERR_ARG, "streams: 1 or 2");
+  if (nstreams == 2 && !P->stream2) {
+    HIP_TRY(hipStreamCreateWithFlags(&P->stream2, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+  }
+  P->nstreams = nstreams;
+  return FAME_OK;
+}
+
+int fame_plan_get_streams(fame_plan* P, int* nstreams) {
+  if (!P || !nstreams) return set_err(FAME_ERR_ARG, "NULL argument");
+  *nstreams = P->nstreams;
   return FAME_OK;
 }
 

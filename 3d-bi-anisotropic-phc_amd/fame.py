@@ -92,6 +92,8 @@ def load_library(path: str = LIB_PATH):
         "fame_plan_set_generic": ([vp, i], i),
         "fame_plan_set_cg_fused": ([vp, i], i),
         "fame_plan_set_cg_graph": ([vp, i], i),
+        "fame_plan_set_streams": ([vp, i], i),
+        "fame_plan_get_streams": ([vp, pi_], i),
         "fame_plan_set_material_tile": ([vp, i, i, i], i),
         "fame_plan_get_material_tile": ([vp, pi_, pi_, pi_], i),
         "fame_plan_set_chunk": ([vp, i], i),
@@ -143,6 +145,7 @@ def exported_symbols():
         "fame_Prs", "fame_Qrs", "fame_fncA", "fame_fncB", "fame_material_apply", "fame_fncinvB", "fame_lanczos",
         "fame_eigen_restoration", "fame_dot", "fame_minres", "fame_block_dot", "fame_block_gemv", "fame_mgs", "fame_block_dot_noalloc", "fame_jd_ritz_residual", "fame_jd_expand",
         "fame_plan_profile", "fame_plan_stage_times", "fame_plan_material_kernel_time", "fame_plan_set_cg_fused", "fame_plan_set_cg_graph",
+        "fame_plan_set_streams", "fame_plan_get_streams",
         "fame_plan_set_material_tile", "fame_plan_get_material_tile",
         "fame_plan_set_generic", "fame_plan_set_chunk", "fame_plan_get_chunk", "fame_plan_autotune",
         "fame_plan_set_variant", "fame_plan_get_variant",
@@ -297,6 +300,15 @@ class Plan:
     def set_cg_graph(self, on=True):
         """Replay the fused CG iteration from a captured HIP graph, or launch one by one (default)."""
         _check(self.lib.fame_plan_set_cg_graph(self.h, int(bool(on))))
+
+    def set_streams(self, nstreams):
+        """Queue fncB / lock-step CG chunks on one stream or alternate them over two."""
+        _check(self.lib.fame_plan_set_streams(self.h, int(nstreams)))
+
+    def streams(self):
+        n = ctypes.c_int()
+        _check(self.lib.fame_plan_get_streams(self.h, ctypes.byref(n)))
+        return n.value
 
     def set_material_tile(self, tx, ty, tz=0):
         """Material-stencil tile (tx, ty columns, tz slices; 0, 0 = heuristic)."""

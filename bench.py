@@ -127,7 +127,7 @@ def traffic_child(pkg, torch, args):
 
 
 def _pmc_per_dispatch(db, counter):
-    """{kernel name: [(grid_z, value per dispatch), ...]} from a rocprofv3 --pmc database."""
+    """{kernel name: [((grid_x, grid_z), value per dispatch), ...]} from a rocprofv3 --pmc database."""
     import sqlite3
 
     c = sqlite3.connect(db)
@@ -135,20 +135,22 @@ def _pmc_per_dispatch(db, counter):
     need = {"kernel_name", "counter_name", "value"}
     if not need <= set(cols):
         raise RuntimeError(f"unexpected counters_collection columns {cols}")
+    gx = "grid_size_x" if "grid_size_x" in cols else ("grid_x" if "grid_x" in cols else None)
     gz = "grid_size_z" if "grid_size_z" in cols else ("grid_z" if "grid_z" in cols else None)
     disp = "dispatch_id" if "dispatch_id" in cols else ("id" if "id" in cols else None)
-    sel = f"select kernel_name, {gz or 1}, {disp or 'rowid'}, value from counters_collection where counter_name = ?"
+    sel = (f"select kernel_name, {gx or 1}, {gz or 1}, {disp or 'rowid'}, value from counters_collection "
+           "where counter_name = ?")
     per = {}
-    for name, z, d, v in c.execute(sel, (counter,)):
-        per.setdefault((name, z, d), 0.0)
-        per[(name, z, d)] += float(v)   # counters may come per XCD / instance: sum per dispatch
+    for name, x, z, d, v in c.execute(sel, (counter,)):
+        per.setdefault((name, (x, z), d), 0.0)
+        per[(name, (x, z), d)] += float(v)   # counters may come per XCD / instance: sum per dispatch
     out = {}
-    for (name, z, d), v in per.items():
-        out.setdefault(name, []).append((z, v))
+    for (name, g, d), v in per.items():
+        out.setdefault(name, []).append((g, v))
     return out
 
 
-def measure_traffic(args, kernel, per_launch, chunk, variant, mtile, timeout=240):
+def measure_traffic(args, kernel, chunk, variant, mtile, timeout=240):
     """HBM bytes per launch of ``kernel`` from two rocprofv3 --pmc runs of ``traffic_child``
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
     for the 1 GiB copy probe in the same run (read 1 GiB, write 1 GiB).  Returns a dict, or None
@@ -176,9 +178,12 @@ def measure_traffic(args, kernel, per_launch, chunk, variant, mtile, timeout=240
                 return None
             per = _pmc_per_dispatch(dbs[0], counter)
             cal = [v for name, vs in per.items() if "k_probe_copy" in name for _, v in vs]
-            hot = [v for name, vs in per.items() if kernel in name for z, v in vs if z == per_launch]
+            # the full-chunk launches: the kernel's most frequent grid (a shorter last chunk differs)
+            grids = [g for name, vs in per.items() if kernel in name for g, _ in vs]
+            full = max(set(grids), key=grids.count) if grids else None
+            hot = [v for name, vs in per.items() if kernel in name for g, v in vs if g == full]
             if not cal or not hot:
-                log(f"bench: {counter}: no dispatches of {kernel} (grid z {per_launch}) or of the copy probe")
+                log(f"bench: {counter}: no dispatches of {kernel} or of the copy probe")
                 return None
             scale = float(1 << 30) / (np.mean(cal) * 1024.0)      # bytes per counted KiB
             res[counter] = {"kib_per_dispatch": float(np.mean(hot)), "copy_probe_kib": float(np.mean(cal)),
@@ -772,6 +777,8 @@ def main():
                     help="fncB matvec rates at n = 24, 48, 60, 72 (batched and plain) with the CPU oracle's")
     ap.add_argument("--chunk", type=int, default=-1,
                     help="k-points per fncB chunk (-1: autotune on the plan, 0: whole batch)")
+    ap.add_argument("--streams", type=int, default=0, choices=[0, 1, 2],
+                    help="with --chunk: fncB chunks on 1 or 2 streams (0: one)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM traffic")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
@@ -833,16 +840,19 @@ def main():
 
     if args.chunk < 0:
         chunk, tune_ms = pb.autotune()
-        log(f"bench: autotuned chunk = {chunk}, xy/z launch variants = {pb.variant()}, material tile = "
-            f"{pb.material_tile()} ({tune_ms:.3f} ms per batched fncB)")
+        log(f"bench: autotuned chunk = {chunk} on {pb.streams()} stream(s), xy/z launch variants = {pb.variant()}, "
+            f"material tile = {pb.material_tile()} ({tune_ms:.3f} ms per batched fncB)")
     else:  # a fixed configuration (no autotuning), e.g. the one autotune picked, for profiling
         chunk = args.chunk
         pb.set_chunk(chunk)
+        if args.streams:
+            pb.set_streams(args.streams)
         if args.variant:
             pb.set_variant(*args.variant)
         if args.mtile:
             pb.set_material_tile(*args.mtile)
-        log(f"bench: fixed chunk = {chunk}, xy/z launch variants = {pb.variant()}, material tile = {pb.material_tile()}")
+        log(f"bench: fixed chunk = {chunk} on {pb.streams()} stream(s), xy/z launch variants = {pb.variant()}, "
+            f"material tile = {pb.material_tile()}")
     x1, y1 = xb[0], yb[0]
     # the plain path at its best: the one-k plan autotuned the same way (launch shapes, tile)
     _, tune1_ms = p1.autotune()
@@ -918,7 +928,7 @@ def main():
     in_profiler = any(k.startswith("ROCPROF") for k in os.environ)
     if not args.no_traffic and not in_profiler and dom in ("material",):
         log("bench: HBM traffic of the dominant kernel (two rocprofv3 --pmc child runs)")
-        tr = measure_traffic(args, TRAFFIC_KERNELS[dom], per_launch, chunk, list(pb.variant()),
+        tr = measure_traffic(args, TRAFFIC_KERNELS[dom], chunk, list(pb.variant()),
                              list(pb.material_tile()))
         if tr is not None:
             roofline["traffic"] = tr["bytes_per_launch"]
@@ -947,8 +957,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"woodpile SC n={n} gamma=0.2, {nk}-k Gamma-X-M-R-Gamma band path, fncB batched over k",
-                   "n": n, "nk": nk, "vector_len": 4 * N, "chunk": chunk,
-                   "fft_variants": list(pb.variant())},
+                   "n": n, "nk": nk, "vector_len": 4 * N, "chunk": chunk, "streams": pb.streams(),
+                   "fft_variants": list(pb.variant()), "material_tile": list(pb.material_tile())},
         "plain_matvecs_per_s": plain_rate,
         "batched_over_plain_matvec": value / plain_rate,
         "fncB_algo_GBs": fncb_gbs,

@@ -20,6 +20,6 @@ def test_help_lists_configs():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0
-    for opt in ("--config", "--chunk", "--variant", "--mtile", "--solve", "--concurrent", "--no-cpu", "--sweep",
-                "--no-traffic"):
+    for opt in ("--config", "--chunk", "--streams", "--variant", "--mtile", "--solve", "--concurrent", "--no-cpu",
+                "--sweep", "--no-traffic"):
         assert opt in r.stdout

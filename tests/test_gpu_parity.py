@@ -466,6 +466,78 @@ def test_material_tiles_identical(pkg, torch_mod, n):
     plan.close()
 
 
+@pytest.mark.parametrize("tile", [(3, 5, 1), (3, 5, 5), (8, 8, 7), (4, 4, 8)])
+def test_material_slabs_with_partial_sums(pkg, torch_mod, tile):
+    """Slab lengths that do not divide n3 (a short last slab), one-slice slabs and whole
+    columns: fncB is the same bit for bit, and the per-work-group partial sums of u^H (Mat u)
+    feed the fused CG (a zero right-hand side is converged before the first pass) and MINRES,
+    which match the textbook CG (1e-10) and the generic stencil's MINRES (1e-12)."""
+    tx, ty, tz = tile
+    say(f"material slabs with partial sums tile={tile}")
+    dims = (8, 8, 8)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE)
+                       for k in ([0.5, 0.3, 0.1], [0.2, 0.0, 0.9], [0.7, 0.7, 0.1])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    x = torch_mod.as_tensor(crand(3, 4 * plan.N))
+    ref = plan.fncB(x).cpu().numpy()
+    plan.set_material_tile(tx, ty, tz)
+    assert np.array_equal(plan.fncB(x).cpu().numpy(), ref)
+    b = x.clone()
+    b[1] = 0
+    yf, itf, rf = plan.fncinvB(b, tol=1e-12, maxit=3000)
+    plan.set_cg_fused(False)
+    yt, itt, rt = plan.fncinvB(b, tol=1e-12, maxit=3000)
+    plan.set_cg_fused(True)
+    yf, yt = yf.cpu().numpy(), yt.cpu().numpy()
+    assert not yf[1].any() and itf[1] == 0
+    for i in (0, 2):
+        assert abs(int(itf[i]) - int(itt[i])) <= 1 and rf[i] <= 1e-12
+        assert rel(yf[i], yt[i]) < 1e-10
+    ym, itm, _ = plan.minres(x, sigma=0.8, tol=1e-30, maxit=6)
+    plan.set_generic(True)
+    yg, itg, _ = plan.minres(x, sigma=0.8, tol=1e-30, maxit=6)
+    for i in range(3):
+        assert itm[i] == itg[i] == 6
+        assert rel(ym[i].cpu().numpy(), yg[i].cpu().numpy()) < 1e-12, i
+    plan.close()
+
+
+@pytest.mark.parametrize("n,chunk", [(24, 2), (12, 1), (60, 4)])
+def test_two_stream_chunks_identical(pkg, torch_mod, n, chunk):
+    """Chunks alternating over two streams (each with its own half of the field scratch) give
+    the one-stream results bit for bit: fncB, the lock-step fused CG (iterations, solution,
+    residuals; a zero right-hand side included), MINRES with the stencil's partial sums and,
+    at n = 12, a batched Lanczos solve."""
+    say(f"two-stream chunks n={n} chunk={chunk}")
+    dims = (n, n, n)
+    S = pkg.structures
+    ks = S.sc_band_path()[[2, 9, 17, 25, 33]]
+    thetas = np.stack([S.theta_of_k(k, S.SC_LATTICE) for k in ks])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    plan.set_chunk(chunk)
+    x = torch_mod.as_tensor(crand(5, 4 * plan.N))
+    x[3] = 0
+    v0 = torch_mod.as_tensor(crand(5, 4 * plan.N))
+    out = {}
+    for ns in (1, 2):
+        plan.set_streams(ns)
+        assert plan.streams() == ns
+        y = plan.fncB(x).cpu().numpy()
+        cg = plan.fncinvB(x, tol=1e-12, maxit=3000)
+        mr = plan.minres(x, sigma=0.8, tol=1e-30, maxit=5)
+        out[ns] = (y, cg[0].cpu().numpy(), cg[1], cg[2], mr[0].cpu().numpy(), mr[2])
+        if n == 12:
+            ev, ew, its = plan.Lanczos(4, tol=1e-10, p=12, v0=v0)
+            out[ns] += (np.asarray(ew), np.asarray(its))
+    for a, b in zip(out[1], out[2]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert int(out[2][2][3]) == 0 and not out[2][1][3].any()
+    plan.close()
+
+
 def test_batched_lanczos_equals_plain(pkg, torch_mod):
     """Lock-step batched Lanczos (k-points finishing at different steps, solved on a
     packed view of the unfinished ones) gives each k-point's plain result."""
