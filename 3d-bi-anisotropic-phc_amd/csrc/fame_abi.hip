@@ -123,12 +123,12 @@ struct fame_plan {
   CgState* h_cg = nullptr;    // pinned host mirror of d_cg for the lagged convergence poll
   int h_cg_cap = 0;
   hipEvent_t ev_poll = nullptr;
-  // second stream for chunk pairs: odd chunks of fncB and of a lock-step CG iteration run on
-  // stream2 with their own half of the field scratch, so one chunk's kernels fill the other's
-  // launch tails; fork / join by events on the caller's stream
+  // extra streams for chunk groups: chunk k of fncB and of a lock-step CG iteration runs on
+  // stream k % nstreams with its own part of the field scratch, so one chunk's kernels fill the
+  // others' launch tails; fork / join by events on the caller's stream
   int nstreams = 1;
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t xstream[2] = {nullptr, nullptr};  // extra streams (chunk k on stream k % nstreams)
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   int w_off = 0;              // items into d_w1 / d_w2 where the current window's fields live
   int tab_len = 0;            // n1+n2+n3
   bool fast = true;           // use the specialised FFT kernels where a size has them
@@ -656,44 +656,49 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
 
 // fncB on all nb items, in chunks so the intermediate fields of a chunk can stay
 // in the device's last-level cache between the five kernels.
-// Two-stream chunk schedule: chunk k runs on the plan's stream (k even) or on stream2 (k odd)
-// with fields in scratch items [(k % 2) c, ...); both streams start after the work queued so far
-// (fork) and the caller's stream waits for stream2 at the end (join).  Chunks touch disjoint items,
-// so the only ordering they need is each chunk's own, which its stream keeps.  Off while
-// profiling (the per-kernel events assume one stream) and for a single chunk.
-struct PairSchedule {
+// Multi-stream chunk schedule: chunk k runs on the plan's stream (k % nstreams == 0) or on
+// extra stream k % nstreams - 1, with fields in scratch items [(k % nstreams) c, ...); all
+// streams start after the work queued so far (fork) and the caller's stream waits for them at
+// the end (join).  Chunks touch disjoint items, so the only ordering they need is each chunk's
+// own, which its stream keeps.  Off while profiling (the per-kernel events assume one stream)
+// and for a single chunk.
+struct ChunkStreams {
   fame_plan* P;
   hipStream_t main;
-  bool on = false;
-  explicit PairSchedule(fame_plan* p) : P(p), main(p->stream) {}
+  int ns = 1;
+  explicit ChunkStreams(fame_plan* p) : P(p), main(p->stream) {}
   int begin(int nchunks) {
-    on = P->nstreams == 2 && !P->prof && nchunks > 1;
-    if (!on) return FAME_OK;
+    ns = (P->nstreams > 1 && !P->prof && nchunks > 1) ? std::min(P->nstreams, nchunks) : 1;
+    if (ns == 1) return FAME_OK;
     HIP_TRY(hipEventRecord(P->ev_fork, main));
-    HIP_TRY(hipStreamWaitEvent(P->stream2, P->ev_fork, 0));
+    for (int i = 0; i + 1 < ns; ++i) HIP_TRY(hipStreamWaitEvent(P->xstream[i], P->ev_fork, 0));
     return FAME_OK;
   }
   void select(int k, int c) {  // before chunk k (chunk size c) is queued
-    if (!on) return;
-    P->stream = (k % 2) ? P->stream2 : main;
-    P->w_off = (k % 2) * c;
+    if (ns == 1) return;
+    const int j = k % ns;
+    P->stream = j ? P->xstream[j - 1] : main;
+    P->w_off = j * c;
   }
-  int end() {  // restores the plan's stream; the caller's stream then waits for stream2
+  int end() {  // restores the plan's stream; the caller's stream then waits for the others
     P->stream = main;
     P->w_off = 0;
-    if (!on) return FAME_OK;
-    on = false;
-    HIP_TRY(hipEventRecord(P->ev_join, P->stream2));
-    HIP_TRY(hipStreamWaitEvent(main, P->ev_join, 0));
+    if (ns == 1) return FAME_OK;
+    const int n = ns;
+    ns = 1;
+    for (int i = 0; i + 1 < n; ++i) {
+      HIP_TRY(hipEventRecord(P->ev_join[i], P->xstream[i]));
+      HIP_TRY(hipStreamWaitEvent(main, P->ev_join[i], 0));
+    }
     return FAME_OK;
   }
-  ~PairSchedule() { (void)end(); }
+  ~ChunkStreams() { (void)end(); }
 };
 
 int do_fncB(fame_plan* P, const double2* x, double2* y, double2* mpart = nullptr, int mpstride = 0) {
   const int nb = P->vnb;
   const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
-  PairSchedule ps(P);
+  ChunkStreams ps(P);
   RET_IF(ps.begin((nb + c - 1) / c));
   for (int b0 = 0, k = 0; b0 < nb; b0 += c, ++k) {
     ps.select(k, c);
@@ -914,7 +919,7 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
       {
         // a chunk's iterations depend only on that chunk: with two streams each keeps its chunks
         // for the whole batch, joined before the state copy below
-        PairSchedule ps(P);
+        ChunkStreams ps(P);
         RET_IF(ps.begin((nb + c - 1) / c));
         for (int k = 0; k < batch; ++k) {
           for (int b0 = 0, ck = 0; b0 < nb; b0 += c, ++ck) {
@@ -1142,10 +1147,12 @@ int fame_plan_destroy(fame_plan* P) {
     if (e) (void)hipEventDestroy(e);
   if (P->ev_poll) (void)hipEventDestroy(P->ev_poll);
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
-  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
-  if (P->stream2) {
-    (void)hipStreamSynchronize(P->stream2);
-    (void)hipStreamDestroy(P->stream2);
+  for (int i = 0; i < 2; ++i) {
+    if (P->ev_join[i]) (void)hipEventDestroy(P->ev_join[i]);
+    if (P->xstream[i]) {
+      (void)hipStreamSynchronize(P->xstream[i]);
+      (void)hipStreamDestroy(P->xstream[i]);
+    }
   }
   if (P->h_cg) (void)hipHostFree(P->h_cg);
   if (P->own_stream) {
@@ -1268,14 +1275,14 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   std::vector<int> cands;
   for (int c : {1, 2, 3, 4, 6, 8, 12, 16})
     if (c < P->nb) cands.push_back(c);
-  // each chunk size on one stream, and on two (chunk pairs overlap) where there are two chunks
+  // each chunk size on one, two or three streams (chunks on different streams overlap)
   P->nstreams = 1;
   int bestns = 1;
   if (!cands.empty()) RET_IF(fame_plan_set_streams(P, 1));
   for (int c : cands)
-    for (int ns = 1; ns <= 2; ++ns) {
-      if (ns == 2) RET_IF(fame_plan_set_streams(P, 2));
-      P->nstreams = ns;
+    for (int ns = 1; ns <= 3; ++ns) {
+      if (ns > 1 && (ns - 1) * c >= P->nb) break;  // fewer chunks than streams
+      RET_IF(fame_plan_set_streams(P, ns));
       P->chunk = c;
       double ms;
       RET_IF(time_it(ms));
@@ -1341,11 +1348,11 @@ int fame_plan_set_cg_graph(fame_plan* P, int on) {
 
 int fame_plan_set_streams(fame_plan* P, int nstreams) {
   if (!P) return set_err(FAME_ERR_ARG, "NULL plan");
-  if (nstreams != 1 && nstreams != 2) return set_err(FAME_ERR_ARG, "streams: 1 or 2");
-  if (nstreams == 2 && !P->stream2) {
-    HIP_TRY(hipStreamCreateWithFlags(&P->stream2, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+  if (nstreams < 1 || nstreams > 3) return set_err(FAME_ERR_ARG, "streams: 1, 2 or 3");
+  if (nstreams > 1 && !P->ev_fork) HIP_TRY(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+  for (int i = 0; i + 1 < nstreams; ++i) {
+    if (!P->xstream[i]) HIP_TRY(hipStreamCreateWithFlags(&P->xstream[i], hipStreamNonBlocking));
+    if (!P->ev_join[i]) HIP_TRY(hipEventCreateWithFlags(&P->ev_join[i], hipEventDisableTiming));
   }
   P->nstreams = nstreams;
   return FAME_OK;

@@ -302,7 +302,7 @@ class Plan:
         _check(self.lib.fame_plan_set_cg_graph(self.h, int(bool(on))))
 
     def set_streams(self, nstreams):
-        """Queue fncB / lock-step CG chunks on one stream or alternate them over two."""
+        """Queue fncB / lock-step CG chunks on one stream or deal them over 2 or 3 streams."""
         _check(self.lib.fame_plan_set_streams(self.h, int(nstreams)))
 
     def streams(self):

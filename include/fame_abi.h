@@ -133,11 +133,11 @@ int fame_plan_set_cg_fused(fame_plan* plan, int fused);
  * Same kernels, same results (no reference counterpart). */
 int fame_plan_set_cg_graph(fame_plan* plan, int on);
 
-/* Chunk schedule on one or two streams (no reference counterpart; same results bit for
- * bit): with nstreams = 2, fncB and each lock-step fused CG iteration queue their odd
- * chunks on a second plan-owned stream with their own half of the field scratch, so one
- * chunk's kernels fill the other's launch tails.  Both streams start after the work already
- * queued on the plan's stream, which waits for both before the call returns its work to it.
+/* Chunk schedule on 1, 2 or 3 streams (no reference counterpart; same results bit for
+ * bit): fncB and each lock-step fused CG iteration queue chunk k on stream k % nstreams
+ * (the plan's stream or a plan-owned extra one) with its own part of the field scratch, so
+ * one chunk's kernels fill the others' launch tails.  Every stream starts after the work
+ * already queued on the plan's stream, which waits for all of them before the call returns.
  * Off while profiling.  fame_plan_autotune chooses it with the chunk size. */
 int fame_plan_set_streams(fame_plan* plan, int nstreams);
 int fame_plan_get_streams(fame_plan* plan, int* nstreams);

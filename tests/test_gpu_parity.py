@@ -505,12 +505,12 @@ def test_material_slabs_with_partial_sums(pkg, torch_mod, tile):
 
 
 @pytest.mark.parametrize("n,chunk", [(24, 2), (12, 1), (60, 4)])
-def test_two_stream_chunks_identical(pkg, torch_mod, n, chunk):
-    """Chunks alternating over two streams (each with its own half of the field scratch) give
-    the one-stream results bit for bit: fncB, the lock-step fused CG (iterations, solution,
-    residuals; a zero right-hand side included), MINRES with the stencil's partial sums and,
-    at n = 12, a batched Lanczos solve."""
-    say(f"two-stream chunks n={n} chunk={chunk}")
+def test_multi_stream_chunks_identical(pkg, torch_mod, n, chunk):
+    """Chunks dealt over two or three streams (each with its own part of the field scratch)
+    give the one-stream results bit for bit: fncB, the lock-step fused CG (iterations,
+    solution, residuals; a zero right-hand side included), MINRES with the stencil's partial
+    sums and, at n = 12, a batched Lanczos solve."""
+    say(f"multi-stream chunks n={n} chunk={chunk}")
     dims = (n, n, n)
     S = pkg.structures
     ks = S.sc_band_path()[[2, 9, 17, 25, 33]]
@@ -522,7 +522,7 @@ def test_two_stream_chunks_identical(pkg, torch_mod, n, chunk):
     x[3] = 0
     v0 = torch_mod.as_tensor(crand(5, 4 * plan.N))
     out = {}
-    for ns in (1, 2):
+    for ns in (1, 2, 3):
         plan.set_streams(ns)
         assert plan.streams() == ns
         y = plan.fncB(x).cpu().numpy()
@@ -532,9 +532,12 @@ def test_two_stream_chunks_identical(pkg, torch_mod, n, chunk):
         if n == 12:
             ev, ew, its = plan.Lanczos(4, tol=1e-10, p=12, v0=v0)
             out[ns] += (np.asarray(ew), np.asarray(its))
-    for a, b in zip(out[1], out[2]):
-        assert np.array_equal(np.asarray(a), np.asarray(b))
-    assert int(out[2][2][3]) == 0 and not out[2][1][3].any()
+    for ns in (2, 3):
+        for a, b in zip(out[1], out[ns]):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), ns
+    assert int(out[3][2][3]) == 0 and not out[3][1][3].any()
+    with pytest.raises(pkg.FameError):
+        plan.set_streams(4)
     plan.close()
 
 
