@@ -1290,7 +1290,8 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
     }
   P->chunk = bestc;
   P->nstreams = bestns;
-  // material stencil tile and z-slab: the heuristic's best few shapes, four slab lengths each
+  // material stencil tile and z-slab: the heuristic's best few shapes (and wide ones), four slab
+  // lengths each
   if (P->fast) {
     struct Cand { double score; int tx, ty; };
     std::vector<Cand> tiles;
@@ -1311,8 +1312,36 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
           for (const Cand& c : tiles) dup |= (c.tx == tx && c.ty == ty);
           if (!dup) tiles.push_back({0.0, tx, ty});
         }
-    const long long nbc = bestc > 0 ? bestc : P->nb;
+    const int nbc = bestc > 0 ? bestc : P->nb;
+    // The tile and slab change only the stencil, so they are chosen by the stencil's own time
+    // on one chunk (steadier than the whole fncB's, where the other four kernels add noise);
+    // the fields it reads are those the fncB runs above left in the scratch.
+    const long long N = P->N;
+    auto time_mat = [&](double& ms_out) -> int {
+      struct Win {
+        fame_plan* P;
+        ~Win() { P->cb0 = 0; P->cnb = P->vnb; }
+      } win{P};
+      P->cb0 = 0;
+      P->cnb = nbc;
+      RET_IF(material(P, P->d_w1, 6 * N, P->d_w2, 6 * N));  // warm-up
+      ms_out = 1e30;
+      for (int trial = 0; trial < 3; ++trial) {
+        HIP_TRY(hipEventRecord(e0, P->stream));
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r) RET_IF(material(P, P->d_w1, 6 * N, P->d_w2, 6 * N));
+        HIP_TRY(hipEventRecord(e1, P->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        ms_out = std::min(ms_out, (double)ms / reps);
+      }
+      return FAME_OK;
+    };
     int btx = 0, bty = 0, btz = 0;
+    double bmat;
+    for (int i = 0; i < 3; ++i) P->mt[i] = 0;
+    RET_IF(time_mat(bmat));  // the heuristic's tile
     // slab lengths: about one, two or three rounds of work-groups, or half a round
     for (const Cand& c : tiles) {
       const int hz = slab_length(P->n1, P->n2, P->n3, c.tx, c.ty, nbc, P->ncu);
@@ -1321,13 +1350,14 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
         P->mt[1] = c.ty;
         P->mt[2] = tz;
         double ms;
-        RET_IF(time_it(ms));
-        if (ms < best) { best = ms; btx = c.tx; bty = c.ty; btz = tz; }
+        RET_IF(time_mat(ms));
+        if (ms < bmat) { bmat = ms; btx = c.tx; bty = c.ty; btz = tz; }
       }
     }
     P->mt[0] = btx;
     P->mt[1] = bty;
     P->mt[2] = btz;
+    RET_IF(time_it(best));  // the whole fncB at the chosen configuration
   }
   G.ok = true;
   if (best_ms) *best_ms = best;

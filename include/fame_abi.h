@@ -93,8 +93,11 @@ int fame_plan_set_stream(fame_plan* plan, void* stream);
 
 /* Batch chunking (no reference counterpart): fncB runs the nb k-points in chunks
  * of `chunk` items (0 = all at once) so a chunk's intermediate fields stay in the
- * last-level cache between kernels.  fame_plan_autotune times the candidate
- * chunk sizes on the plan and keeps the fastest (best_ms may be NULL). */
+ * last-level cache between kernels.  fame_plan_autotune times, on the plan, the FFT
+ * launch shapes, then the chunk size with the stream count (whole fncB, best of three
+ * timings), then the material-stencil tile and slab (the stencil's own time on one
+ * chunk), keeps the fastest of each and returns the fncB time of the result in
+ * best_ms (which may be NULL). */
 int fame_plan_set_chunk(fame_plan* plan, int chunk);
 int fame_plan_get_chunk(fame_plan* plan);
 int fame_plan_autotune(fame_plan* plan, double* best_ms);
