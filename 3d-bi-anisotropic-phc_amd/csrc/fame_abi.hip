@@ -130,6 +130,10 @@ struct fame_plan {
   hipStream_t xstream[2] = {nullptr, nullptr};  // extra streams (chunk k on stream k % nstreams)
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   int w_off = 0;              // items into d_w1 / d_w2 where the current window's fields live
+  // pinned host staging for the small device-to-host reads a call waits for (solver states,
+  // inner products): a copy into pageable memory goes through the runtime's own staging
+  void* h_stage = nullptr;
+  size_t h_stage_cap = 0;
   int tab_len = 0;            // n1+n2+n3
   bool fast = true;           // use the specialised FFT kernels where a size has them
   // batch window used by the launch helpers: items [cb0, cb0 + cnb) of the current view
@@ -789,10 +793,26 @@ int copy(fame_plan* P, const double2* x, double2* y, long long len) {
   return FAME_OK;
 }
 
+// dst (host) <- src (device), `bytes` of them, through the plan's pinned staging buffer; waits
+// for the plan's stream
+int read_to_host(fame_plan* P, void* dst, const void* src, size_t bytes) {
+  if (bytes > P->h_stage_cap) {
+    if (P->h_stage) HIP_TRY(hipHostFree(P->h_stage));
+    P->h_stage = nullptr;
+    P->h_stage_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 16384);
+    HIP_TRY(hipHostMalloc(&P->h_stage, cap, hipHostMallocDefault));
+    P->h_stage_cap = cap;
+  }
+  HIP_TRY(hipMemcpyAsync(P->h_stage, src, bytes, hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  std::memcpy(dst, P->h_stage, bytes);
+  return FAME_OK;
+}
+
 int cg_finish(fame_plan* P, std::vector<CgState>& hs, int* iters, double* relres) {
   const int nb = P->vnb;
-  HIP_TRY(hipMemcpyAsync(hs.data(), P->d_cg, nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
-  HIP_TRY(hipStreamSynchronize(P->stream));
+  RET_IF(read_to_host(P, hs.data(), P->d_cg, nb * sizeof(CgState)));
   bool all = true;
   for (int i = 0; i < nb; ++i) {
     if (iters) iters[i] = (int)hs[i].iters;
@@ -845,8 +865,7 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
   };
   // all items of [b0, b0 + nbc) converged?
   auto finished = [&](int b0, int nbc, bool& done) -> int {
-    HIP_TRY(hipMemcpyAsync(hs.data() + b0, P->d_cg + b0, nbc * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
-    HIP_TRY(hipStreamSynchronize(P->stream));
+    RET_IF(read_to_host(P, hs.data() + b0, P->d_cg + b0, nbc * sizeof(CgState)));
     done = true;
     for (int i = b0; i < b0 + nbc; ++i)
       if (hs[i].active != 0.0) done = false;
@@ -983,8 +1002,7 @@ int do_cg_plain(fame_plan* P, const double2* b, double2* x, double tol, int maxi
     CHECK_LAUNCH();
     ++it;
     if (it % check_every == 0 || it == maxit) {
-      HIP_TRY(hipMemcpyAsync(hs.data(), P->d_cg, nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
-      HIP_TRY(hipStreamSynchronize(P->stream));
+      RET_IF(read_to_host(P, hs.data(), P->d_cg, nb * sizeof(CgState)));
       done = true;
       for (int i = 0; i < nb; ++i)
         if (hs[i].active != 0.0) done = false;
@@ -1155,6 +1173,7 @@ int fame_plan_destroy(fame_plan* P) {
     }
   }
   if (P->h_cg) (void)hipHostFree(P->h_cg);
+  if (P->h_stage) (void)hipHostFree(P->h_stage);
   if (P->own_stream) {
     (void)hipStreamSynchronize(P->own_stream);
     (void)hipStreamDestroy(P->own_stream);
