@@ -94,6 +94,9 @@ struct fame_plan {
   void* d_mrst = nullptr;     // MINRES per-item state [2][nb] (the iteration alternates halves)
   double2* d_mrp = nullptr;   // MINRES partial sums [3][nb][2048] and alpha [nb]
   double2* d_mrm = nullptr;   // MINRES: the material pass's partial sums of u^H A u [nb][mrm_stride]
+  CgState* d_mrskip = nullptr;  // MINRES: per-item active flags for the fncB passes [nb]
+  void* h_mr = nullptr;         // MINRES: pinned copy of the state for the lagged poll [nb]
+  hipEvent_t ev_mr = nullptr;
   size_t mrm_cap = 0;
   double2* d_mgp = nullptr;   // fame_mgs partial sums [2][1024]
   double2* d_mgs = nullptr;   // fame_mgs coefficients when the caller passes none
@@ -345,10 +348,10 @@ int xy_fast_launch(fame_plan* P, int dir, int twist, const double2* in, long lon
 
 template <int N, int A, int B, int kZTP, int NT>
 int z_fast_launch(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs,
-                  int nslot, int grp_base, double sc, const CgFuse* cf) {
+                  int nslot, int grp_base, double sc, const CgFuse* cf, const CgState* skip) {
   const dim3 grid((P->geo.n12 + kZTP - 1) / kZTP, nslot, P->cnb);
   const size_t lds = (size_t)(3 * N * kZTP + N) * sizeof(double2);
-  const CgFuse c0{nullptr, nullptr, nullptr, nullptr, 0};
+  const CgFuse c0{const_cast<CgState*>(skip), nullptr, nullptr, nullptr, 0};
   if (dir > 0) {
     if (mode == ZM_MIX_IN && cf)
       hipLaunchKernelGGL((k_zfast<N, A, B, 1, ZM_MIX_IN, kZTP, NT, 1>), grid, dim3(NT), lds, P->stream, P->geo, in,
@@ -414,16 +417,17 @@ int set_fast_attrs(int n) {
 }
 
 // ---- launch helpers -------------------------------------------------------
-// cf (fast z kernels only): fuse the CG updates into the MIX_IN / MIX_OUT pass
+// cf (fast z kernels only): fuse the CG updates into the MIX_IN / MIX_OUT pass; skip (fast
+// kernels, without cf): items whose state is inactive are left alone
 int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, double2* out, long long obs, int nslot,
-          int grp_base, const CgFuse* cf = nullptr) {
+          int grp_base, const CgFuse* cf = nullptr, const CgState* skip = nullptr) {
   const double sc = 1.0 / std::sqrt((double)P->n3);
   if (P->fast && use_fast_z(P)) {
 #define X(N, A, B)                                                                                       \
     if (P->n3 == N) {                                                                                      \
-      if (P->zv == 1) return z_fast_launch<N, A, B, 8, 128>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf); \
-      if (P->zv == 2) return z_fast_launch<N, A, B, 32, 512>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf); \
-      return z_fast_launch<N, A, B, 16, 256>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf);        \
+      if (P->zv == 1) return z_fast_launch<N, A, B, 8, 128>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip); \
+      if (P->zv == 2) return z_fast_launch<N, A, B, 32, 512>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip); \
+      return z_fast_launch<N, A, B, 16, 256>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip);        \
     }
     FAME_FAST_SIZES(X)
 #undef X
@@ -592,7 +596,7 @@ int twist(fame_plan* P, const double2* in, double2* out, int ncomp, int conj_twi
 // mpart (optional, without cf): the material pass leaves its per-work-group partial sums of
 // u^H (Mat u) there, item stride mpstride (for an isometric transform chain this is x^H fncB x)
 int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, const CgFuse* cf = nullptr,
-                double2* mpart = nullptr, int mpstride = 0) {
+                double2* mpart = nullptr, int mpstride = 0, const CgState* skip = nullptr) {
   const long long N = P->N;
   struct Window {  // restores the full-batch window on every exit path
     fame_plan* P;
@@ -618,10 +622,12 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   // MIX_OUT pass updates r in place (y is r then)
   double2* w1 = P->d_w1 + (size_t)P->w_off * 6 * N;
   double2* w2 = P->d_w2 + (size_t)P->w_off * 6 * N;
-  RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, w1, 6 * N, 2, 0, pcf));
+  // items a caller's iteration has finished (skip, without cf) are left alone by every pass
+  const CgState* wskip = (!pcf && skip) ? skip + b0 : nullptr;
+  RET_IF(zpass(P, +1, ZM_MIX_IN, x, 4 * N, w1, 6 * N, 2, 0, pcf, wskip));
   if (pr) HIP_TRY(hipEventRecord(P->ev[1], P->stream));
   // converged items of a fused CG are skipped by every pass (their state is frozen)
-  const CgState* cgst = pcf ? wcf.st : nullptr;
+  const CgState* cgst = pcf ? wcf.st : wskip;
   // the xy passes run in place (each work-group loads its whole plane before its first barrier
   // and stores it after its last): a chunk's fields then occupy two buffers that are each read
   // and rewritten on the same lines, which keeps more of them in the Infinity Cache between
@@ -638,7 +644,7 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
   RET_IF(xypass(P, -1, 0, w2, 6 * N, w2, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
-  RET_IF(zpass(P, -1, ZM_MIX_OUT, w2, 6 * N, y, 4 * N, 2, 0, pcf));
+  RET_IF(zpass(P, -1, ZM_MIX_OUT, w2, 6 * N, y, 4 * N, 2, 0, pcf, wskip));
   if (pr) {
     HIP_TRY(hipEventRecord(P->ev[5], P->stream));
     HIP_TRY(hipEventSynchronize(P->ev[5]));
@@ -699,14 +705,16 @@ struct ChunkStreams {
   ~ChunkStreams() { (void)end(); }
 };
 
-int do_fncB(fame_plan* P, const double2* x, double2* y, double2* mpart = nullptr, int mpstride = 0) {
+// skip (optional, [vnb]): items whose .active is 0 are left alone (specialised kernels)
+int do_fncB(fame_plan* P, const double2* x, double2* y, double2* mpart = nullptr, int mpstride = 0,
+            const CgState* skip = nullptr) {
   const int nb = P->vnb;
   const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
   ChunkStreams ps(P);
   RET_IF(ps.begin((nb + c - 1) / c));
   for (int b0 = 0, k = 0; b0 < nb; b0 += c, ++k) {
     ps.select(k, c);
-    RET_IF(fncB_window(P, b0, std::min(c, nb - b0), x, y, nullptr, mpart, mpstride));
+    RET_IF(fncB_window(P, b0, std::min(c, nb - b0), x, y, nullptr, mpart, mpstride, skip));
   }
   RET_IF(ps.end());
   if (P->prof) P->prof_calls += 1;
@@ -1156,7 +1164,7 @@ int fame_plan_destroy(fame_plan* P) {
   void* bufs[] = {P->d_tw, P->d_perm, P->d_lam, P->d_tws, P->d_omega, P->d_mat_id, P->d_mat, P->d_w1, P->d_w2,
                   P->d_part, P->d_red, P->d_r, P->d_p, P->d_q, P->d_cg, P->d_lam_v, P->d_tws_v, P->d_om_v,
                   P->d_idx, P->d_part2, P->d_eps, P->d_gam, P->d_pvw, P->d_mr, P->d_mrst, P->d_mgs, P->d_bdp, P->d_jdy, P->d_mrp,
-                  P->d_mgp, P->d_mrm};
+                  P->d_mgp, P->d_mrm, P->d_mrskip};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t e : P->ev)
@@ -1174,6 +1182,8 @@ int fame_plan_destroy(fame_plan* P) {
   }
   if (P->h_cg) (void)hipHostFree(P->h_cg);
   if (P->h_stage) (void)hipHostFree(P->h_stage);
+  if (P->h_mr) (void)hipHostFree(P->h_mr);
+  if (P->ev_mr) (void)hipEventDestroy(P->ev_mr);
   if (P->own_stream) {
     (void)hipStreamSynchronize(P->own_stream);
     (void)hipStreamDestroy(P->own_stream);

@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
 //           array and updates it in place from each output q, r <- r - alpha q (q is not
 //           stored), leaving one partial sum of ||r||^2 (in .x) per (item, slot, work-group)
 //           in cf.part[item * cf.nparts + slot * gridDim.x + blockIdx.x].
+// CG = 0 uses only cf.st, and only to skip items whose state is inactive (nullptr: none).
 struct CgFuse {
   CgState* st;
   double2* x;
@@ -202,8 +203,9 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
   const int slot = blockIdx.y;
   const int grp = grp_base + slot;
   const int b = blockIdx.z;
-  // a converged CG item is left as it is: its last deferred x update is applied by k_cg_xfin
-  if (CG && cf.st[b].active == 0.0) return;
+  // a converged CG item is left as it is: its last deferred x update is applied by k_cg_xfin;
+  // outside the CG, cf.st (when given) marks items a caller's iteration has finished
+  if (cf.st && cf.st[b].active == 0.0) return;
   const double2* __restrict__ lb = lam + (long long)b * lam_bstride;
   const double2* __restrict__ twn = twg + g.ax[2].tw_off;
   for (int t = threadIdx.x; t < N3; t += NT) tw[t] = twn[t];

@@ -185,6 +185,39 @@ def test_minres_converges(pkg, fo, torch_mod):
     plan.close()
 
 
+@pytest.mark.parametrize("n", [8, 24])
+def test_minres_finished_items_frozen(pkg, fo, torch_mod, n):
+    """MINRES polls its state one batch of iterations behind, so iterations are queued after an
+    item has converged; every pass then leaves the item alone.  A batch with a zero right-hand
+    side (finished before the first iteration) and two items that converge at different steps:
+    each item's iteration count and solution equal those of the item solved alone (within one
+    iteration and 1e-9: the lone plan's stencil sums its partials in other groups), the zero
+    item stays zero, and the solutions solve the shifted system to the tolerance."""
+    say(f"minres finished items frozen n={n}")
+    dims = (n, n, n)
+    thetas = np.vstack([THETAS, [[0.9, 0.6, -1.9]]])
+    plan, probs = woodpile_plan(pkg, fo, dims, thetas)
+    x = crand(3, 4 * plan.N)
+    x[0] = 0
+    x[1] *= 1e-3   # the same tolerance relative to ||b|| converges at a different step per item
+    sigma, tol = 0.3, 1e-10
+    y, iters, relres = plan.minres(torch_mod.as_tensor(x), sigma=sigma, tol=tol, maxit=4000)
+    y = y.cpu().numpy()
+    assert iters[0] == 0 and not y[0].any()
+    for b in (1, 2):
+        p1, _ = woodpile_plan(pkg, fo, dims, thetas[b])
+        y1, it1, _ = p1.minres(torch_mod.as_tensor(x[b]), sigma=sigma, tol=tol, maxit=4000)
+        assert abs(int(it1[0]) - int(iters[b])) <= 1, (b, it1, iters)
+        assert rel(y[b], y1.cpu().numpy()) < 1e-9, b
+        p1.close()
+        if n == 8:
+            P = probs[b]
+            r = P.fncB(y[b]) - sigma * P.fncA(y[b]) - x[b]
+            assert np.linalg.norm(r) <= 10 * tol * np.linalg.norm(x[b])
+        assert relres[b] <= tol
+    plan.close()
+
+
 def test_jdsira_woodpile_matches_dense(pkg, fo, torch_mod):
     say("JD-SIRA woodpile n=6 vs dense GEP")
     dims = (6, 6, 6)
