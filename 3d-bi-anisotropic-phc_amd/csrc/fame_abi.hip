@@ -614,12 +614,14 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
     wcf.x = cf->x + (size_t)b0 * 4 * N;
     wcf.p = cf->p + (size_t)b0 * 4 * N;
     wcf.part = cf->part + (size_t)b0 * cf->nparts;
+    wcf.part_r = cf->part_r + (size_t)b0 * cf->nparts;
+    wcf.nparts_m = material_parts(P, nbc);
     pcf = &wcf;
   }
   const bool pr = P->prof;
   if (pr) HIP_TRY(hipEventRecord(P->ev[0], P->stream));
-  // CG: the material pass sums p^H q (in real space), alpha is formed right after it, and the
-  // MIX_OUT pass updates r in place (y is r then)
+  // CG: the material pass sums p^H q (in real space), the MIX_OUT pass forms alpha from those
+  // sums and updates r in place (y is r then)
   double2* w1 = P->d_w1 + (size_t)P->w_off * 6 * N;
   double2* w2 = P->d_w2 + (size_t)P->w_off * 6 * N;
   // items a caller's iteration has finished (skip, without cf) are left alone by every pass
@@ -636,11 +638,6 @@ int fncB_window(fame_plan* P, int b0, int nbc, const double2* x, double2* y, con
   if (pr) HIP_TRY(hipEventRecord(P->ev[2], P->stream));
   double2* mp = pcf ? wcf.part : (mpart ? mpart + (size_t)b0 * mpstride : nullptr);
   RET_IF(material(P, w1, 6 * N, w2, 6 * N, mp, pcf ? wcf.nparts : mpstride, cgst));
-  if (pcf) {
-    hipLaunchKernelGGL(k_cg_alpha2, dim3(nbc), dim3(kBlk), 0, P->stream, wcf.part, material_parts(P, nbc), wcf.nparts,
-                       wcf.st);
-    CHECK_LAUNCH();
-  }
   if (pr) HIP_TRY(hipEventRecord(P->ev[3], P->stream));
   RET_IF(xypass(P, -1, 0, w2, 6 * N, w2, 6 * N, 6, cgst));
   if (pr) HIP_TRY(hipEventRecord(P->ev[4], P->stream));
@@ -833,43 +830,42 @@ int cg_finish(fame_plan* P, std::vector<CgState>& hs, int* iters, double* relres
 
 int zfast_tp(const fame_plan* P) { return P->zv == 1 ? 8 : (P->zv == 2 ? 32 : 16); }
 
-// CG with its vector updates fused into fncB (same iterates as the unfused loop below up
-// to rounding): per chunk of items, fncB(p) with p <- r + beta p and the deferred
-// x <- x + alpha p done in the first pass and p^H q summed in the last, then alpha,
-// r <- r - alpha q with ||r||^2, and beta -- all while the chunk's vectors are recent in
-// the last-level cache.  x receives its last alpha p after the loop.
+// CG with its vector updates and scalar steps fused into fncB (same iterates as the unfused
+// loop below up to rounding): per chunk of items, fncB(p) where the first pass forms beta from
+// the previous residual's partial sums, tests convergence, and does p <- r + beta p and the
+// deferred x <- x + alpha p; the material pass sums p^H q; the last pass forms alpha and does
+// r <- r - alpha q with the partials of ||r||^2 -- five kernels per chunk-iteration, while the
+// chunk's vectors are recent in the last-level cache.  After the loop k_cg_fin2 tests the last
+// residual and x receives its last alpha p.
 int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxit, int* iters, double* relres) {
   const long long L = 4 * P->N;
   const long long tot = L * P->vnb;
   const int nb = P->vnb;
   // per-item partial sums: p^H q from the material pass (one per work-group of a launch of nbc
-  // items), then ||r||^2 from the MIX_OUT pass (one per z work-group and slot) in the same slots
+  // items) in the first area, ||r||^2 from the MIX_OUT pass (one per z work-group and slot) in
+  // the second; the per-item stride is the largest count of any launch
   const int nparts_z = 2 * ((P->geo.n12 + zfast_tp(P) - 1) / zfast_tp(P));
-  int nparts = nparts_z;  // per-item stride: the largest count of any launch
+  int nparts = nparts_z;
   for (int c = 1; c <= nb; ++c) nparts = std::max(nparts, material_parts(P, c));
-  if (P->part2_cap < (size_t)P->nb * nparts) {
+  if (P->part2_cap < (size_t)2 * P->nb * nparts) {
     if (P->d_part2) HIP_TRY(hipFree(P->d_part2));
     P->d_part2 = nullptr;
     P->part2_cap = 0;
-    HIP_TRY(hipMalloc(&P->d_part2, (size_t)P->nb * nparts * sizeof(double2)));
-    P->part2_cap = (size_t)P->nb * nparts;
+    HIP_TRY(hipMalloc(&P->d_part2, (size_t)2 * P->nb * nparts * sizeof(double2)));
+    P->part2_cap = (size_t)2 * P->nb * nparts;
   }
+  double2* part_r = P->d_part2 + (size_t)P->nb * nparts;
   RET_IF(fill_zero(P, x, tot));
   RET_IF(copy(P, b, P->d_r, tot));
   RET_IF(fill_zero(P, P->d_p, tot));
   RET_IF(do_dot(P, b, L, b, L, L, nb, P->d_red));
   hipLaunchKernelGGL(k_cg_init, dim3((nb + 63) / 64), dim3(64), 0, P->stream, P->d_red, P->d_cg, nb, tol);
   CHECK_LAUNCH();
-  const CgFuse cf{P->d_cg, x, P->d_p, P->d_part2, nparts};
   std::vector<CgState> hs(nb);
-  // one iteration on the chunk [b0, b0 + nbc): alpha and r <- r - alpha q inside the pass
-  // chain (the output of fncB is r itself), then beta from ||r||^2
-  auto step = [&](int b0, int nbc) -> int {
-    RET_IF(fncB_window(P, b0, nbc, P->d_r, P->d_r, &cf));
-    hipLaunchKernelGGL(k_cg_beta, dim3(nbc), dim3(kBlk), 0, P->stream, P->d_part2 + (size_t)b0 * nparts, nparts_z,
-                       nparts, P->d_cg + b0);
-    CHECK_LAUNCH();
-    return FAME_OK;
+  // iteration `it` on the chunk [b0, b0 + nbc) (the output of fncB is r itself)
+  auto step = [&](int b0, int nbc, int it) -> int {
+    const CgFuse cf{P->d_cg, x, P->d_p, P->d_part2, nparts, part_r, 0, it & 1, it == 0 ? 1 : 0};
+    return fncB_window(P, b0, nbc, P->d_r, P->d_r, &cf);
   };
   // all items of [b0, b0 + nbc) converged?
   auto finished = [&](int b0, int nbc, bool& done) -> int {
@@ -879,48 +875,57 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
       if (hs[i].active != 0.0) done = false;
     return FAME_OK;
   };
+  const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
   if (P->cg_graph && !P->prof && P->cg_seq == 0) {
-    // one iteration over all chunks captured once as a HIP graph on the plan's own stream and
-    // replayed on the plan's stream (the pointers do not change within the solve)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipStream_t saved = P->stream;
-    HIP_TRY(hipStreamBeginCapture(P->own_stream, hipStreamCaptureModeThreadLocal));
-    P->stream = P->own_stream;
-    const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
+    // iteration 0 eagerly, then iterations 1 and 2 over all chunks captured once as a HIP graph
+    // on the plan's own stream (odd parity, then even) and replayed on the plan's stream: the
+    // pointers and parities do not change within the solve
     int rc = FAME_OK;
-    for (int b0 = 0; b0 < nb && rc == FAME_OK; b0 += c) rc = step(b0, std::min(c, nb - b0));
-    P->stream = saved;
-    hipError_t ec = hipStreamEndCapture(P->own_stream, &graph);
-    if (rc != FAME_OK) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc;
-    }
-    if (ec != hipSuccess) return set_err(FAME_ERR_HIP, std::string("CG graph capture: ") + hipGetErrorString(ec));
-    ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ec != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return set_err(FAME_ERR_HIP, std::string("CG graph instantiate: ") + hipGetErrorString(ec));
-    }
-    bool done = false;
-    for (int it = 0; it < maxit && !done && rc == FAME_OK;) {
-      ec = hipGraphLaunch(exec, P->stream);
-      if (ec != hipSuccess) rc = set_err(FAME_ERR_HIP, std::string("CG graph launch: ") + hipGetErrorString(ec));
-      ++it;
-      if (rc == FAME_OK && (it % 4 == 0 || it == maxit)) rc = finished(0, nb, done);
-    }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    for (int b0 = 0; b0 < nb && rc == FAME_OK; b0 += c) rc = step(b0, std::min(c, nb - b0), 0);
     RET_IF(rc);
+    int it = 1;
+    bool done = false;
+    if (maxit >= 3) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      hipStream_t saved = P->stream;
+      HIP_TRY(hipStreamBeginCapture(P->own_stream, hipStreamCaptureModeThreadLocal));
+      P->stream = P->own_stream;
+      for (int k = 1; k <= 2 && rc == FAME_OK; ++k)
+        for (int b0 = 0; b0 < nb && rc == FAME_OK; b0 += c) rc = step(b0, std::min(c, nb - b0), k);
+      P->stream = saved;
+      hipError_t ec = hipStreamEndCapture(P->own_stream, &graph);
+      if (rc != FAME_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (ec != hipSuccess) return set_err(FAME_ERR_HIP, std::string("CG graph capture: ") + hipGetErrorString(ec));
+      ec = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ec != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return set_err(FAME_ERR_HIP, std::string("CG graph instantiate: ") + hipGetErrorString(ec));
+      }
+      while (it + 2 <= maxit && !done && rc == FAME_OK) {
+        ec = hipGraphLaunch(exec, P->stream);
+        if (ec != hipSuccess) rc = set_err(FAME_ERR_HIP, std::string("CG graph launch: ") + hipGetErrorString(ec));
+        it += 2;
+        if (rc == FAME_OK && ((it - 1) % 4 == 0 || it + 2 > maxit)) rc = finished(0, nb, done);
+      }
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      RET_IF(rc);
+    }
+    if (!done && it < maxit)  // an odd remainder of maxit
+      for (int b0 = 0; b0 < nb; b0 += c) RET_IF(step(b0, std::min(c, nb - b0), it));
   } else if (P->cg_seq > 0) {
     // chunk by chunk to convergence: a chunk's vectors and fncB intermediates stay in the
     // last-level cache for all its iterations
-    const int c = std::min(P->cg_seq, nb);
-    for (int b0 = 0; b0 < nb; b0 += c) {
-      const int nbc = std::min(c, nb - b0);
+    const int cs = std::min(P->cg_seq, nb);
+    for (int b0 = 0; b0 < nb; b0 += cs) {
+      const int nbc = std::min(cs, nb - b0);
       bool done = false;
       for (int it = 0; it < maxit && !done;) {
-        RET_IF(step(b0, nbc));
+        RET_IF(step(b0, nbc, it));
         ++it;
         if (it % 4 == 0 || it == maxit) RET_IF(finished(b0, nbc, done));
       }
@@ -939,7 +944,6 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
       P->h_cg_cap = nb;
     }
     if (!P->ev_poll) HIP_TRY(hipEventCreateWithFlags(&P->ev_poll, hipEventDisableTiming));
-    const int c = (P->chunk > 0 && P->chunk < nb) ? P->chunk : nb;
     bool done = false, pending = false;
     for (int it = 0; it < maxit && !done;) {
       const int batch = std::min(4, maxit - it);
@@ -951,7 +955,7 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
         for (int k = 0; k < batch; ++k) {
           for (int b0 = 0, ck = 0; b0 < nb; b0 += c, ++ck) {
             ps.select(ck, c);
-            RET_IF(step(b0, std::min(c, nb - b0)));
+            RET_IF(step(b0, std::min(c, nb - b0), it + k));
           }
           if (P->prof) P->prof_calls += 1;
         }
@@ -971,6 +975,8 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
       }
     }
   }
+  hipLaunchKernelGGL(k_cg_fin2, dim3(nb), dim3(kBlk), 0, P->stream, part_r, nparts_z, nparts, P->d_cg);
+  CHECK_LAUNCH();
   hipLaunchKernelGGL(k_cg_xfin, dim3(grid1d(L), nb), dim3(kBlk), 0, P->stream, x, P->d_p, L, P->d_cg);
   CHECK_LAUNCH();
   return cg_finish(P, hs, iters, relres);

@@ -172,20 +172,31 @@ __global__ void __launch_bounds__(NT) k_xyfast(Geo g, const double2* __restrict_
 // z-pencil pass (n3 = N3 = A*B), TP consecutive pencils per work-group.
 // LDS: [3][N3][TP] (pencil index fastest) + twiddles (N3).
 // ---------------------------------------------------------------------------
-// CG = 1 fuses the conjugate-gradient vector updates into the first and last pass of fncB:
-//   MIX_IN  reads r (the input), p and x at each point, forms p <- r + beta p and the
-//           deferred x <- x + alpha p_old, writes p and x back and transforms p;
-//   MIX_OUT (alpha is known: p^H q was summed by the material pass) holds r in its output
+// CG = 1 fuses the conjugate-gradient iteration into the first and last pass of fncB:
+//   MIX_IN  (iteration i >= 1) sums the previous MIX_OUT pass's partials of ||r||^2 into
+//           rho_i, forms beta = rho_i / rho_{i-1} and tests convergence; every work-group
+//           does this in the same order, so all reach the same values, and work-group 0
+//           stores rho_i (slot i & 1), counts the iteration and clears the item's active flag
+//           when it has converged (the item then returns at once).  Then it reads r (the
+//           input), p and x at each point, forms p <- r + beta p and the deferred
+//           x <- x + alpha p_old, writes p and x back and transforms p;
+//   MIX_OUT sums the material pass's partials of p^H q, forms alpha = rho_i / p^H q (work-group
+//           0 stores it for the next MIX_IN and the final x update), holds r in its output
 //           array and updates it in place from each output q, r <- r - alpha q (q is not
 //           stored), leaving one partial sum of ||r||^2 (in .x) per (item, slot, work-group)
-//           in cf.part[item * cf.nparts + slot * gridDim.x + blockIdx.x].
+//           in cf.part_r[item * cf.nparts + slot * gridDim.x + blockIdx.x].
+// No separate scalar kernel runs between the passes.
 // CG = 0 uses only cf.st, and only to skip items whose state is inactive (nullptr: none).
 struct CgFuse {
   CgState* st;
   double2* x;
   double2* p;
-  double2* part;
+  double2* part;    // material partials of p^H q, item stride nparts
   int nparts;
+  double2* part_r;  // MIX_OUT partials of ||r||^2, item stride nparts
+  int nparts_m;     // material partials per item in this launch
+  int par;          // iteration parity: the rho slot of this iteration
+  int first;        // iteration 0: beta = 0, no residual partials yet
 };
 
 template <int N3, int A, int B, int DIR, int MODE, int TP, int NT, int CG = 0>
@@ -220,8 +231,22 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
     if (CG) {
       pf = cf.p + b * in_bstride + (long long)(2 * slot) * N;
       xf = cf.x + b * in_bstride + (long long)(2 * slot) * N;
-      cal = cf.st[b].alpha;
-      cbe = cf.st[b].beta;
+      CgState* __restrict__ st = cf.st + b;
+      cal = st->alpha;
+      if (!cf.first) {
+        const double rprev = cf.par ? st->rho : st->rho1;  // rho_{i-1}, not written here
+        const double bn2 = st->bnorm2, t2 = st->tol2, its = st->iters;
+        const double rn = part_sum_all(cf.part_r + (long long)b * cf.nparts, gridDim.x * gridDim.y).x;
+        cbe = rn / rprev;
+        const bool conv = rn <= t2 * bn2;
+        if (blockIdx.x == 0 && slot == 0 && threadIdx.x == 0) {
+          if (cf.par) st->rho1 = rn;
+          else st->rho = rn;
+          st->iters = its + 1;
+          if (conv) st->active = 0.0;
+        }
+        if (conv) return;  // uniform: every work-group reached the same rn
+      }
     }
     for (int it = threadIdx.x; it < N3 * TP; it += NT) {
       const int k3 = it / TP, pp = it - (it / TP) * TP;
@@ -329,7 +354,12 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
     }
     __syncthreads();
     double2* __restrict__ yout = out + b * out_bstride + (long long)(2 * slot) * N;
-    const double ca = CG ? cf.st[b].alpha : 0.0;
+    double ca = 0.0;
+    if (CG) {  // alpha = rho_i / p^H q from the material pass's partials (same in every work-group)
+      const double pq = part_sum_all(cf.part + (long long)b * cf.nparts, cf.nparts_m).x;
+      ca = (cf.par ? cf.st[b].rho1 : cf.st[b].rho) / pq;
+      if (blockIdx.x == 0 && slot == 0 && threadIdx.x == 0) cf.st[b].alpha = ca;
+    }
     double2 dacc = make_double2(0, 0);
     for (int it = threadIdx.x; it < N3 * TP; it += NT) {
       const int k3 = it / TP, pp = it - (it / TP) * TP;
@@ -365,7 +395,7 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
     }
     if (CG) {
       const double2 s = block_sum(dacc);
-      if (threadIdx.x == 0) cf.part[(long long)b * cf.nparts + slot * gridDim.x + blockIdx.x] = s;
+      if (threadIdx.x == 0) cf.part_r[(long long)b * cf.nparts + slot * gridDim.x + blockIdx.x] = s;
     }
   } else {
     // phase B straight to HBM: item (c, k, pp); output z = k + A q

@@ -407,6 +407,23 @@ __device__ __forceinline__ double2 block_sum(double2 v) {
   return v;  // valid in thread 0
 }
 
+// block_sum with the result in every thread (uniform control flow)
+__device__ __forceinline__ double2 block_sum_all(double2 v) {
+  __shared__ double2 bc;
+  v = block_sum(v);
+  if (threadIdx.x == 0) bc = v;
+  __syncthreads();
+  return bc;
+}
+
+// sum of n partial sums, in every thread: thread t adds entries t, t + blockDim.x, ... in order,
+// then block_sum (with 256 threads the order of k_cg_beta, so the same bits)
+__device__ __forceinline__ double2 part_sum_all(const double2* __restrict__ part, int n) {
+  double2 acc = make_double2(0, 0);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc = cadd(acc, part[i]);
+  return block_sum_all(acc);
+}
+
 // part[b*gridDim.x + blk] = sum over this block's range of conj(x) .* y
 __global__ void __launch_bounds__(256) k_dot_part(const double2* __restrict__ x, long long xb,
                                                   const double2* __restrict__ y, long long yb, long long len,
@@ -442,11 +459,13 @@ __global__ void __launch_bounds__(256) k_reduce_part(const double2* __restrict__
 // ---------------------------------------------------------------------------
 // Conjugate gradients on B (fncinvB), all batch items in lock step.
 // Scalars per batch item live on the device:
-//   cg[b] = {rho, bnorm2, alpha, beta, active, iters, pad, pad}
+//   cg[b] = {rho, bnorm2, alpha, beta, active, iters, tol2, rho1}
+// The fused CG keeps rho of even iterations in rho and of odd ones in rho1, so the pass that
+// forms the new rho never overwrites the value its other work-groups are still reading.
 // ---------------------------------------------------------------------------
 struct CgState {
   double rho, bnorm2, alpha, beta;
-  double active, iters, tol2, pad;
+  double active, iters, tol2, rho1;
 };
 
 __global__ void k_cg_init(const double2* __restrict__ rr, CgState* __restrict__ st, int nb, double tol) {
@@ -460,7 +479,7 @@ __global__ void k_cg_init(const double2* __restrict__ rr, CgState* __restrict__ 
   s.active = (s.bnorm2 > 0) ? 1.0 : 0.0;
   s.iters = 0;
   s.tol2 = tol * tol;
-  s.pad = 0;
+  s.rho1 = 0;
   st[b] = s;
 }
 
@@ -536,20 +555,24 @@ __global__ void k_cg_p(double2* __restrict__ p, const double2* __restrict__ r, l
   }
 }
 
-// Fused-CG scalar steps (the p / x updates run inside the first pass of fncB, the p^H q
-// partial sums inside the material pass and the r update inside the last pass):
-// alpha = rho / Re(p^H q) from nparts partial sums per item (item stride `stride`)
-__global__ void __launch_bounds__(256) k_cg_alpha2(const double2* __restrict__ part, int nparts, int stride,
-                                                   CgState* __restrict__ st) {
+// End of a fused CG whose scalar steps ran inside the z passes: the MIX_IN pass of iteration
+// i >= 1 forms rho_i from the previous MIX_OUT partials, tests convergence and counts the
+// iteration; this does the same for the residual the last queued iteration left (items still
+// active), and moves every item's latest rho (slot iters & 1) into rho for the host.
+__global__ void __launch_bounds__(256) k_cg_fin2(const double2* __restrict__ part, int nblk, int stride,
+                                                 CgState* __restrict__ st) {
   const int b = blockIdx.x;
-  double2 acc = make_double2(0, 0);
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc = cadd(acc, part[(long long)b * stride + i]);
-  acc = block_sum(acc);
+  const double2 acc = part_sum_all(part + (long long)b * stride, nblk);
   if (threadIdx.x == 0) {
     CgState s = st[b];
-    // a converged item keeps its last alpha: the fused passes skip it from then on, and
-    // k_cg_xfin applies its deferred update x += alpha p once at the end
-    if (s.active != 0.0) s.alpha = s.rho / acc.x;
+    if (s.active != 0.0) {
+      s.rho = acc.x;
+      s.iters += 1;
+      if (acc.x <= s.tol2 * s.bnorm2) s.active = 0.0;
+    } else if (((long long)s.iters) & 1) {
+      s.rho = s.rho1;
+    }
+    s.beta = 0.0;
     st[b] = s;
   }
 }

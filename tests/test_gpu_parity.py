@@ -247,6 +247,41 @@ def test_cg_sequential_chunks_match_lock_step(pkg, torch_mod):
     plan.close()
 
 
+@pytest.mark.parametrize("maxit", [1, 2, 5, 6])
+def test_fused_cg_stops_at_maxit_like_textbook(pkg, torch_mod, maxit):
+    """Cut off at maxit, the fused CG (its scalar steps inside the z passes, the last residual
+    tested after the loop) returns the textbook CG's iterate, iteration count and residual, on
+    the lock-step, graph-replayed and sequential-chunk schedules."""
+    say(f"fused CG at maxit={maxit} == textbook")
+    dims = (12, 12, 12)
+    S = pkg.structures
+    thetas = np.stack([S.theta_of_k(np.pi * np.array(k), S.SC_LATTICE)
+                       for k in ([0.5, 0.3, 0.1], [0.2, 0.0, 0.9], [0.7, 0.7, 0.1])])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    plan.set_chunk(2)
+    x = torch_mod.as_tensor(crand(3, 4 * plan.N)).to("cuda")
+    x[1] = 0
+    plan.set_cg_fused(False)
+    yt, itt, rt = plan.fncinvB(x, tol=1e-12, maxit=maxit, raise_on_maxit=False)
+    plan.set_cg_fused(True)
+    runs = [plan.fncinvB(x, tol=1e-12, maxit=maxit, raise_on_maxit=False)]
+    plan.set_cg_graph(True)
+    runs.append(plan.fncinvB(x, tol=1e-12, maxit=maxit, raise_on_maxit=False))
+    plan.set_cg_graph(False)
+    plan.set_cg_fused(True, seq_chunk=2)
+    runs.append(plan.fncinvB(x, tol=1e-12, maxit=maxit, raise_on_maxit=False))
+    yt = yt.cpu().numpy()
+    for yf, itf, rf in runs:
+        yf = yf.cpu().numpy()
+        assert list(itf) == [maxit, 0, maxit] and list(itt) == [maxit, 0, maxit]
+        assert not yf[1].any()
+        for b in (0, 2):
+            assert rel(yf[b], yt[b]) < 1e-12
+            assert abs(rf[b] - rt[b]) <= 1e-9 * rt[b]
+    plan.close()
+
+
 def test_cg_graph_replays_match_eager(pkg, torch_mod):
     """The fused CG replayed from a captured HIP graph gives the eagerly launched result on
     every call, with other device work and synchronisations between the calls."""
