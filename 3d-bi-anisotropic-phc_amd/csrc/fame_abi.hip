@@ -154,7 +154,8 @@ struct fame_plan {
   int* d_idx = nullptr;       // [nb] item indices of the packed view
   int chunk = 0;              // fncB processes the batch in chunks of this many items (0: all)
   int xyv = 0;                // xy-pass launch shape: 0 = default threads, 1 = 512 threads (default from n1 >= 40)
-  int zv = 0;                 // z-pass launch shape: 0 = 16 pencils/256 thr, 1 = 8/128, 2 = 32/512
+  int zv = 0;                 // z-pass launch shape: 0 = 16 pencils/256 thr, 1 = 8/128, 2 = 32/512, 3 = 16/320,
+                              // 4 = 16/384
   // per-stage timing of fncB (HIP events between the five kernels), off by default
   bool prof = false;
   hipEvent_t ev[6] = {};
@@ -406,6 +407,8 @@ int fast_set_attrs() {
   RET_IF((z_attrs<N, A, B, 16, 256>()));
   RET_IF((z_attrs<N, A, B, 8, 128>()));
   RET_IF((z_attrs<N, A, B, 32, 512>()));
+  RET_IF((z_attrs<N, A, B, 16, 320>()));
+  RET_IF((z_attrs<N, A, B, 16, 384>()));
   return FAME_OK;
 }
 
@@ -427,6 +430,8 @@ int zpass(fame_plan* P, int dir, int mode, const double2* in, long long ibs, dou
     if (P->n3 == N) {                                                                                      \
       if (P->zv == 1) return z_fast_launch<N, A, B, 8, 128>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip); \
       if (P->zv == 2) return z_fast_launch<N, A, B, 32, 512>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip); \
+      if (P->zv == 3) return z_fast_launch<N, A, B, 16, 320>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip); \
+      if (P->zv == 4) return z_fast_launch<N, A, B, 16, 384>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip); \
       return z_fast_launch<N, A, B, 16, 256>(P, dir, mode, in, ibs, out, obs, nslot, grp_base, sc, cf, skip);        \
     }
     FAME_FAST_SIZES(X)
@@ -828,7 +833,7 @@ int cg_finish(fame_plan* P, std::vector<CgState>& hs, int* iters, double* relres
   return FAME_OK;
 }
 
-int zfast_tp(const fame_plan* P) { return P->zv == 1 ? 8 : (P->zv == 2 ? 32 : 16); }
+int zfast_tp(const fame_plan* P) { return P->zv == 1 ? 8 : (P->zv == 2 ? 32 : 16); }  // zv 3, 4: 16 pencils
 
 // CG with its vector updates and scalar steps fused into fncB (same iterates as the unfused
 // loop below up to rounding): per chunk of items, fncB(p) where the first pass forms beta from
@@ -1222,7 +1227,7 @@ int fame_plan_set_chunk(fame_plan* P, int chunk) {
 int fame_plan_get_chunk(fame_plan* P) { return P ? P->chunk : -1; }
 
 int fame_plan_set_variant(fame_plan* P, int xy_variant, int z_variant) {
-  if (!P || xy_variant < 0 || xy_variant > 1 || z_variant < 0 || z_variant > 2) return set_err(FAME_ERR_ARG, "bad variant");
+  if (!P || xy_variant < 0 || xy_variant > 1 || z_variant < 0 || z_variant > 4) return set_err(FAME_ERR_ARG, "bad variant");
   P->xyv = xy_variant;
   P->zv = z_variant;
   return FAME_OK;
@@ -1298,7 +1303,7 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   // launch shapes first (whole batch), then the chunk size with the best shapes
   P->chunk = 0;
   for (int xv = 0; xv < 2; ++xv)
-    for (int zv = 0; zv < 3; ++zv) {
+    for (int zv = 0; zv < 5; ++zv) {
       P->xyv = xv;
       P->zv = zv;
       double ms;

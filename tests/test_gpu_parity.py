@@ -460,7 +460,7 @@ def test_launch_variants_identical(pkg, torch_mod, n):
     x = torch_mod.as_tensor(crand(3, 4 * plan.N))
     ref = plan.fncB(x).cpu().numpy()
     for xv in (0, 1):
-        for zv in (0, 1, 2):
+        for zv in (0, 1, 2, 3, 4):
             plan.set_variant(xv, zv)
             assert np.array_equal(plan.fncB(x).cpu().numpy(), ref), (xv, zv)
     plan.close()
