@@ -153,7 +153,8 @@ struct fame_plan {
   double2* d_om_v = nullptr;
   int* d_idx = nullptr;       // [nb] item indices of the packed view
   int chunk = 0;              // fncB processes the batch in chunks of this many items (0: all)
-  int xyv = 0;                // xy-pass launch shape: 0 = default threads, 1 = 512 threads (default from n1 >= 40)
+  int xyv = 0;                // xy-pass launch shape: 0 = default threads, 1 = 512 threads (default from n1 >= 40),
+                              // 2 = 1024 threads for planes of n1 >= 64 (512 below)
   int zv = 0;                 // z-pass launch shape: 0 = 16 pencils/256 thr, 1 = 8/128, 2 = 32/512, 3 = 16/320,
                               // 4 = 16/384
   // per-stage timing of fncB (HIP events between the five kernels), off by default
@@ -404,6 +405,7 @@ template <int N, int A, int B>
 int fast_set_attrs() {
   RET_IF((xy_attrs<N, A, B, xy_threads<N>()>()));
   RET_IF((xy_attrs<N, A, B, 512>()));
+  if constexpr (N >= 64) RET_IF((xy_attrs<N, A, B, 1024>()));
   RET_IF((z_attrs<N, A, B, 16, 256>()));
   RET_IF((z_attrs<N, A, B, 8, 128>()));
   RET_IF((z_attrs<N, A, B, 32, 512>()));
@@ -466,7 +468,8 @@ int xypass(fame_plan* P, int dir, int twist, const double2* in, long long ibs, d
   if (P->fast && use_fast_xy(P)) {
 #define X(N, A, B)                                                                                    \
     if (P->n1 == N) {                                                                                   \
-      if (P->xyv == 1) return xy_fast_launch<N, A, B, 512>(P, dir, twist, in, ibs, out, obs, nf, sc, cgst); \
+      if (P->xyv == 1 || (P->xyv == 2 && N < 64)) return xy_fast_launch<N, A, B, 512>(P, dir, twist, in, ibs, out, obs, nf, sc, cgst); \
+      if (P->xyv == 2) return xy_fast_launch<N, A, B, (N >= 64 ? 1024 : 512)>(P, dir, twist, in, ibs, out, obs, nf, sc, cgst); \
       return xy_fast_launch<N, A, B, xy_threads<N>()>(P, dir, twist, in, ibs, out, obs, nf, sc, cgst);      \
     }
     FAME_FAST_SIZES(X)
@@ -1227,7 +1230,7 @@ int fame_plan_set_chunk(fame_plan* P, int chunk) {
 int fame_plan_get_chunk(fame_plan* P) { return P ? P->chunk : -1; }
 
 int fame_plan_set_variant(fame_plan* P, int xy_variant, int z_variant) {
-  if (!P || xy_variant < 0 || xy_variant > 1 || z_variant < 0 || z_variant > 4) return set_err(FAME_ERR_ARG, "bad variant");
+  if (!P || xy_variant < 0 || xy_variant > 2 || z_variant < 0 || z_variant > 4) return set_err(FAME_ERR_ARG, "bad variant");
   P->xyv = xy_variant;
   P->zv = z_variant;
   return FAME_OK;
@@ -1302,7 +1305,7 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   int bestc = 0, bestxy = 0, bestz = 0;
   // launch shapes first (whole batch), then the chunk size with the best shapes
   P->chunk = 0;
-  for (int xv = 0; xv < 2; ++xv)
+  for (int xv = 0; xv < (P->n1 >= 64 ? 3 : 2); ++xv)
     for (int zv = 0; zv < 5; ++zv) {
       P->xyv = xv;
       P->zv = zv;

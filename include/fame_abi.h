@@ -103,7 +103,8 @@ int fame_plan_get_chunk(fame_plan* plan);
 int fame_plan_autotune(fame_plan* plan, double* best_ms);
 
 /* Launch shapes of the specialised FFT passes (no reference counterpart):
- * xy_variant 0 = default threads per plane work-group, 1 = 512 threads;
+ * xy_variant 0 = default threads per plane work-group, 1 = 512 threads, 2 = 1024 threads for planes of
+ * n1 >= 64 (512 below);
  * z_variant 0 = 16 pencils / 256 threads, 1 = 8 / 128, 2 = 32 / 512, 3 = 16 / 320, 4 = 16 / 384.
  * fame_plan_autotune picks these first, then the chunk size. */
 int fame_plan_set_variant(fame_plan* plan, int xy_variant, int z_variant);

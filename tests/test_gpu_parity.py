@@ -448,7 +448,7 @@ def test_chunked_fncB_equals_whole_batch(pkg, torch_mod):
     plan.close()
 
 
-@pytest.mark.parametrize("n", [24, 60])
+@pytest.mark.parametrize("n", [24, 60, 72])
 def test_launch_variants_identical(pkg, torch_mod, n):
     """Every FFT launch-shape variant computes the same numbers (bitwise)."""
     say(f"launch variants n={n}")
@@ -459,7 +459,7 @@ def test_launch_variants_identical(pkg, torch_mod, n):
     plan = pkg.Plan(dims, thetas, table, ids)
     x = torch_mod.as_tensor(crand(3, 4 * plan.N))
     ref = plan.fncB(x).cpu().numpy()
-    for xv in (0, 1):
+    for xv in (0, 1, 2):
         for zv in (0, 1, 2, 3, 4):
             plan.set_variant(xv, zv)
             assert np.array_equal(plan.fncB(x).cpu().numpy(), ref), (xv, zv)
