@@ -614,6 +614,11 @@ def run_config4(pkg, torch, args):
         with torch.cuda.stream(streams[i]):
             plans.append(pkg.Plan(dims, theta, S.bi_isotropic_table(13.0, gm, A), ids))
             plans[-1].autotune()  # launch shapes and material tile of the one-k plan (outside the timing)
+    # untimed warm-up: one loose eigenpair, so the first timed solve does not pay the one-off loading of
+    # the solver kernels and host libraries (about 0.3 s otherwise)
+    with torch.cuda.stream(streams[0]):
+        pkg.jdsira.GEP_AB_Herm_JDSIRA_Driver4(plans[0], 1, stop_tolerance=1e-3, target=args.target, target_type="CL",
+                                              sigma_inner=args.target, seed=0)
     torch.cuda.synchronize()
     runs, errs = [None] * len(cases), []
 
