@@ -21,5 +21,5 @@ def test_help_lists_configs():
                        timeout=120)
     assert r.returncode == 0
     for opt in ("--config", "--chunk", "--streams", "--variant", "--mtile", "--solve", "--concurrent", "--no-cpu",
-                "--sweep", "--no-traffic"):
+                "--sweep", "--no-traffic", "--sequential"):
         assert opt in r.stdout

@@ -80,3 +80,18 @@ def test_lanczos_reports_unconverged_items(pkg, torch_mod):
     ev, ew, it = plan.Lanczos(4, tol=1e-10, maxiter=10000, raise_on_maxit=False)
     assert np.all(it > 0)
     plan.close()
+
+
+def test_lanczos_refuses_bad_inner_solver_arguments(pkg, torch_mod):
+    """cg_maxit < 1 or cg_tol <= 0 is an argument error, raised before any inner solve."""
+    print("Lanczos refuses cg_maxit = 0 and cg_tol = 0", flush=True)
+    dims = (6, 6, 6)
+    S = pkg.structures
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    plan = pkg.Plan(dims, theta, S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), S.woodpile_sc_ids(dims))
+    for kw in ({"cg_maxit": 0}, {"cg_tol": 0.0}):
+        with pytest.raises(pkg.FameError):
+            plan.Lanczos(2, tol=1e-8, maxiter=10, **kw)
+    ev, ew, it = plan.Lanczos(2, tol=1e-8, maxiter=1000)  # the plan still works afterwards
+    assert np.all(it > 0)
+    plan.close()
