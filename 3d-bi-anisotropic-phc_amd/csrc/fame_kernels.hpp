@@ -619,6 +619,31 @@ __global__ void k_tsgemm(const double2* __restrict__ Q, long long qbs, long long
   }
 }
 
+// Q(:, 0:k) <- Q(:, 0:m) * U(0:m, 0:k) in place (real U, column-major, leading dim ldu; k <= m <= MM):
+// each thread reads all m values of its row into registers before it writes any, so the product
+// needs no scratch copy.  Same summation order as k_tsgemm, so the same bits.
+template <int MM>
+__global__ void __launch_bounds__(256) k_tsgemm_ip(double2* __restrict__ Q, long long len, const double* __restrict__ U,
+                                                   int ldu, int m, int k) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len; i += (long long)gridDim.x * blockDim.x) {
+    double2 q[MM];
+#pragma unroll
+    for (int l = 0; l < MM; ++l) q[l] = l < m ? Q[l * len + i] : make_double2(0, 0);
+    for (int j = 0; j < k; ++j) {
+      double2 acc = make_double2(0, 0);
+#pragma unroll
+      for (int l = 0; l < MM; ++l) {
+        if (l < m) {
+          const double u = U[j * ldu + l];
+          acc.x = fma(u, q[l].x, acc.x);
+          acc.y = fma(u, q[l].y, acc.y);
+        }
+      }
+      Q[j * len + i] = acc;
+    }
+  }
+}
+
 // y = S .* x (conj_twist = 0) or conj(S) .* x (conj_twist = 1), ncomp components of N points.
 __global__ void k_twist(Geo g, const double2* __restrict__ x, double2* __restrict__ y, int ncomp,
                         const double2* __restrict__ tws, int tws_bstride, int conj_twist) {
