@@ -95,3 +95,22 @@ def test_lanczos_refuses_bad_inner_solver_arguments(pkg, torch_mod):
     ev, ew, it = plan.Lanczos(2, tol=1e-8, maxiter=1000)  # the plan still works afterwards
     assert np.all(it > 0)
     plan.close()
+
+
+def test_lanczos_wide_restart_matches_narrow(pkg, torch_mod):
+    """A restart basis wider than 32 columns (p = 34) forms Q U through a scratch product instead
+    of in place; it converges to the same eigenvalues as p = 20 (1e-10), with small residuals."""
+    print("Lanczos p = 34 (scratch restart product) vs p = 20 (in place), n=6", flush=True)
+    dims = (6, 6, 6)
+    S = pkg.structures
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    plan = pkg.Plan(dims, theta, S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), S.woodpile_sc_ids(dims))
+    _, ew20, _ = plan.Lanczos(12, tol=1e-12, maxiter=10000, p=20)
+    ev34, ew34, it34 = plan.Lanczos(12, tol=1e-12, maxiter=10000, p=34)
+    assert it34[0] > 34  # at least one restart
+    assert np.allclose(np.sort(ew34[0]), np.sort(ew20[0]), rtol=1e-10, atol=0)
+    for j in range(12):
+        u = ev34[0, j].contiguous()
+        r = plan.fncB(u) - float(1.0 / ew34[0, j]) * plan.fncA(u)
+        assert float(r.norm() / u.norm()) < 1e-8
+    plan.close()
