@@ -1322,15 +1322,27 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   P->nstreams = 1;
   int bestns = 1;
   if (!cands.empty()) RET_IF(fame_plan_set_streams(P, 1));
-  for (int c : cands)
+  // chunk sizes in increasing order, each on one, two or three streams; the fncB time is unimodal
+  // in both (measured at n = 24-96: it falls to the best chunk and stream count and rises after),
+  // so a stream count slower than the one before ends that chunk's stream search, and two chunk
+  // sizes in a row more than 5 % slower than the best found end the chunk search
+  int worse = 0;
+  for (int c : cands) {
+    double cbest = 1e30, prev = 1e30;
     for (int ns = 1; ns <= 3; ++ns) {
       if (ns > 1 && (ns - 1) * c >= P->nb) break;  // fewer chunks than streams
       RET_IF(fame_plan_set_streams(P, ns));
       P->chunk = c;
       double ms;
       RET_IF(time_it(ms));
+      cbest = std::min(cbest, ms);
       if (ms < best) { best = ms; bestc = c; bestns = ns; }
+      if (ms > prev) break;
+      prev = ms;
     }
+    worse = (cbest > 1.05 * best) ? worse + 1 : 0;
+    if (worse >= 2) break;
+  }
   P->chunk = bestc;
   P->nstreams = bestns;
   // material stencil tile and z-slab: the heuristic's best few shapes (and wide ones), four slab
