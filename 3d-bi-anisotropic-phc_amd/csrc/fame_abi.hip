@@ -1305,14 +1305,24 @@ int fame_plan_autotune(fame_plan* P, double* best_ms) {
   int bestc = 0, bestxy = 0, bestz = 0;
   // launch shapes first (whole batch), then the chunk size with the best shapes
   P->chunk = 0;
-  for (int xv = 0; xv < (P->n1 >= 64 ? 3 : 2); ++xv)
-    for (int zv = 0; zv < 5; ++zv) {
-      P->xyv = xv;
-      P->zv = zv;
-      double ms;
-      RET_IF(time_it(ms));
-      if (ms < best) { best = ms; bestxy = xv; bestz = zv; }
-    }
+  // the xy and z shapes belong to different kernels, whose times add: the xy shape with the
+  // plan's z shape, then the z shape with the best xy shape
+  const int z0 = P->zv;
+  for (int xv = 0; xv < (P->n1 >= 64 ? 3 : 2); ++xv) {
+    P->xyv = xv;
+    P->zv = z0;
+    double ms;
+    RET_IF(time_it(ms));
+    if (ms < best) { best = ms; bestxy = xv; bestz = z0; }
+  }
+  P->xyv = bestxy;
+  for (int zv = 0; zv < 5; ++zv) {
+    if (zv == z0) continue;
+    P->zv = zv;
+    double ms;
+    RET_IF(time_it(ms));
+    if (ms < best) { best = ms; bestz = zv; }
+  }
   P->xyv = bestxy;
   P->zv = bestz;
   std::vector<int> cands;
