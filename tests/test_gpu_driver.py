@@ -114,3 +114,42 @@ def test_lanczos_wide_restart_matches_narrow(pkg, torch_mod):
         r = plan.fncB(u) - float(1.0 / ew34[0, j]) * plan.fncA(u)
         assert float(r.norm() / u.norm()) < 1e-8
     plan.close()
+
+
+def test_distinct_plans_from_threads_match_sequential(pkg, torch_mod):
+    """Plans are independent handles: two plans (different media) driven at once from two host
+    threads, each on its own stream, give the results they give one after another, bit for bit
+    (Lanczos with its fused CG, and a JD-SIRA solve)."""
+    import threading
+
+    print("two plans from two threads == sequential, n=8", flush=True)
+    dims = (8, 8, 8)
+    S = pkg.structures
+    theta = S.theta_of_k(np.pi * np.array([0.5, 0.3, 0.1]), S.SC_LATTICE)
+    ids = S.woodpile_sc_ids(dims)
+    tables = [S.bi_isotropic_table(13.0, 0.2, S.SC_LATTICE), S.bi_isotropic_table(9.0, 0.5, S.SC_LATTICE)]
+
+    def solve(i, out, stream):
+        with torch_mod.cuda.stream(stream):
+            pl = pkg.Plan(dims, theta, tables[i], ids)
+            ev, ew, it = pl.Lanczos(4, tol=1e-10, maxiter=10000, p=8)
+            r = pkg.jdsira.GEP_AB_Herm_JDSIRA_Driver4(pl, 3, stop_tolerance=1e-9, target=2.0, target_type="CL",
+                                                       sigma_inner=2.0, seed=0)
+            stream.synchronize()
+            out[i] = (ew.copy(), it.copy(), np.sort(r.ew.real), r.outer_iterations)
+            pl.close()
+
+    streams = [torch_mod.cuda.Stream() for _ in tables]
+    seq = [None, None]
+    for i in range(2):
+        solve(i, seq, streams[i])
+    par = [None, None]
+    th = [threading.Thread(target=solve, args=(i, par, streams[i])) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in range(2):
+        assert par[i] is not None
+        assert np.array_equal(par[i][0], seq[i][0]) and np.array_equal(par[i][1], seq[i][1])
+        assert np.array_equal(par[i][2], seq[i][2]) and par[i][3] == seq[i][3]
