@@ -100,7 +100,8 @@ TRAFFIC_KERNELS = {"zpass_mix_in": "k_zfast", "xypass_inv": "k_xyfast", "materia
 
 def traffic_child(pkg, torch, args):
     """Run under rocprofv3 --pmc by measure_traffic: two batched fncB calls at the configuration
-    the parent chose, then the 1 GiB copy probe whose known byte count calibrates the counters."""
+    the parent chose (with --traffic-cg: six iterations of the fused CG on every item instead),
+    then the 1 GiB copy probe whose known byte count calibrates the counters."""
     S = pkg.structures
     n, nk = args.n, args.nk
     dims = (n, n, n)
@@ -115,8 +116,12 @@ def traffic_child(pkg, torch, args):
         pb.set_material_tile(*args.mtile)
     rng = np.random.default_rng(1234)
     x = torch.as_tensor(rng.standard_normal((nk, 4 * n ** 3)) + 1j * rng.standard_normal((nk, 4 * n ** 3))).cuda()
-    for _ in range(2):
-        pb.fncB(x)
+    if args.traffic_cg:
+        pb.set_streams(args.streams or 1)
+        pb.fncinvB(x, tol=1e-12, maxit=6, raise_on_maxit=False)
+    else:
+        for _ in range(2):
+            pb.fncB(x)
     torch.cuda.synchronize()
     a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
@@ -792,6 +797,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure the dominant kernel's HBM traffic")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-cg", action="store_true",
+                    help="with --traffic-child: six fused-CG iterations instead of two fncB calls")
     args = ap.parse_args()
 
     if args.gpus != 1:
