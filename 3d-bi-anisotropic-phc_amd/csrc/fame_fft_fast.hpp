@@ -215,8 +215,17 @@ __global__ void __launch_bounds__(NT) k_zfast(Geo g, const double2* __restrict__
   const int grp = grp_base + slot;
   const int b = blockIdx.z;
   // a converged CG item is left as it is: its last deferred x update is applied by k_cg_xfin;
-  // outside the CG, cf.st (when given) marks items a caller's iteration has finished
-  if (cf.st && cf.st[b].active == 0.0) return;
+  // outside the CG, cf.st (when given) marks items a caller's iteration has finished.
+  // The flag is read once per work-group: in the CG's MIX_IN pass work-group 0 clears it when
+  // the item converges while other work-groups of the same launch are starting, and waves that
+  // read it on either side of that store would split their work-group, leaving the waves that
+  // go on to reduce the partial sums below with slots of the exited waves unwritten.
+  if (cf.st) {
+    __shared__ int skip_wg;
+    if (threadIdx.x == 0) skip_wg = cf.st[b].active == 0.0;
+    __syncthreads();
+    if (skip_wg) return;
+  }
   const double2* __restrict__ lb = lam + (long long)b * lam_bstride;
   const double2* __restrict__ twn = twg + g.ax[2].tw_off;
   for (int t = threadIdx.x; t < N3; t += NT) tw[t] = twn[t];

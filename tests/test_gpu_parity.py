@@ -282,6 +282,33 @@ def test_fused_cg_stops_at_maxit_like_textbook(pkg, torch_mod, maxit):
     plan.close()
 
 
+def test_fused_cg_two_streams_repeatable(pkg, torch_mod):
+    """The lock-step fused CG with its chunks dealt over two streams gives the same bits when a
+    solve is repeated, over many right-hand sides whose items converge at different iterations.
+    Regression guard: the z pass read an item's converged flag once per wave, so a work-group
+    whose waves read it around work-group 0's store split, and the waves that went on reduced
+    with unwritten slots (about one solve in a thousand went wrong, more often with two streams,
+    whose concurrent kernels stretch wave start times).  A guard with limited power: that race
+    showed itself rarely."""
+    say("fused CG, two streams, repeated solves bit for bit")
+    dims = (48, 48, 48)
+    S = pkg.structures
+    krng = np.random.default_rng(11)
+    thetas = np.stack([S.theta_of_k(np.pi * krng.uniform(-1, 1, 3), S.SC_LATTICE) for _ in range(24)])
+    table, ids = woodpile(pkg, dims)
+    plan = pkg.Plan(dims, thetas, table, ids)
+    plan.set_chunk(3)
+    plan.set_streams(2)
+    for r in range(40):
+        x = torch_mod.as_tensor(crand(24, 4 * plan.N)).to("cuda")
+        y0, it0, _ = plan.fncinvB(x, tol=1e-12, maxit=2000)
+        y0 = y0.clone()
+        y1, it1, _ = plan.fncinvB(x, tol=1e-12, maxit=2000)
+        assert list(it1) == list(it0), f"right-hand side {r}"
+        assert torch_mod.equal(y0, y1), f"right-hand side {r}"
+    plan.close()
+
+
 def test_cg_graph_replays_match_eager(pkg, torch_mod):
     """The fused CG replayed from a captured HIP graph gives the eagerly launched result on
     every call, with other device work and synchronisations between the calls."""
