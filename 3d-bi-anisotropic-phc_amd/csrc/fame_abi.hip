@@ -943,7 +943,10 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
     // one batch of 4 iterations behind: after each batch the state goes to pinned host memory
     // and is read once the next batch is queued, so the device never waits for the host.
     // Iterations queued after the last item converged find every item inactive and return
-    // at once (converged items are skipped by every pass).
+    // at once (converged items are skipped by every pass).  Once every unconverged item's
+    // ||r||^2 is within 1e4 of its threshold the batches shrink to one iteration, so fewer such
+    // empty iterations are queued; results do not depend on the batch sizes (a converged item's
+    // state is frozen on the device).
     if (P->h_cg_cap < nb) {
       if (P->h_cg) HIP_TRY(hipHostFree(P->h_cg));
       P->h_cg = nullptr;
@@ -953,8 +956,9 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
     }
     if (!P->ev_poll) HIP_TRY(hipEventCreateWithFlags(&P->ev_poll, hipEventDisableTiming));
     bool done = false, pending = false;
+    int bsize = 4;
     for (int it = 0; it < maxit && !done;) {
-      const int batch = std::min(4, maxit - it);
+      const int batch = std::min(bsize, maxit - it);
       {
         // a chunk's iterations depend only on that chunk: with two streams each keeps its chunks
         // for the whole batch, joined before the state copy below
@@ -973,8 +977,16 @@ int do_cg_fused(fame_plan* P, const double2* b, double2* x, double tol, int maxi
       if (pending) {  // the state after the previous batch
         HIP_TRY(hipEventSynchronize(P->ev_poll));
         done = true;
-        for (int i = 0; i < nb; ++i)
-          if (P->h_cg[i].active != 0.0) done = false;
+        bool near = true;
+        for (int i = 0; i < nb; ++i) {
+          const CgState& s = P->h_cg[i];
+          if (s.active == 0.0) continue;
+          done = false;
+          // rho_i sits in the slot of i's parity, and iters = i after MIX_IN pass i
+          const double rn = ((long long)s.iters & 1) ? s.rho1 : s.rho;
+          if (!(rn <= 1e4 * s.tol2 * s.bnorm2)) near = false;
+        }
+        if (near) bsize = 1;
       }
       if (!done) {
         HIP_TRY(hipMemcpyAsync(P->h_cg, P->d_cg, (size_t)nb * sizeof(CgState), hipMemcpyDeviceToHost, P->stream));
